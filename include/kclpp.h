@@ -1,0 +1,200 @@
+/*
+ * kclpp.h - C ABI of the MI355X Sim-kClist++ library (libkclpp.so).
+ *
+ * The library replaces the reference's data-parallel hot path: kClist
+ * k-clique enumeration over the degeneracy-ordered DAG fused with the
+ * Sim-kClist++ update (every k-clique gives one unit to its member of
+ * smallest rho, ties to the smallest id), the PAVA-preprocess pass, the
+ * densest-prefix report, and the k-clique degree pass of core trimming.
+ * The enumeration runs as hand-written HIP kernels on one gfx950 GPU; the
+ * sequential parts whose order defines the results (the degeneracy heap,
+ * the upper-bound scan, the trimming peel, max-flow) run on the host.
+ *
+ * Conventions: plain pointers and sizes, no globals.  Output arrays are
+ * caller-allocated host memory.  Every call returns a kcl_status; on
+ * failure kcl_last_error() holds a message for the calling thread.  A
+ * kcl_graph / kcl_sim is used by one host thread at a time; separate
+ * handles are independent.  Calls are synchronous.
+ *
+ * Reference citations are to btsun/kclistpp.
+ */
+#ifndef KCLPP_H
+#define KCLPP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum kcl_status {
+  KCL_OK = 0,
+  KCL_E_IO = 1,          /* a file cannot be opened, read or written */
+  KCL_E_PARSE = 2,       /* malformed input */
+  KCL_E_INVALID_ARG = 3, /* NULL handle, k out of range, self-loop, ... */
+  KCL_E_OOM_HOST = 4,
+  KCL_E_OOM_DEVICE = 5,
+  KCL_E_HIP = 6,         /* HIP runtime failure or no usable GPU */
+  KCL_E_OVERFLOW = 7,    /* a count reached 2^53, where the reference's
+                            double counters stop being exact */
+  KCL_E_UNSUPPORTED = 8  /* input outside the library's limits */
+} kcl_status;
+
+/* Task of one enumeration pass (kCDensestCmpSync.c:128). */
+typedef enum kcl_task {
+  KCL_FRANK_WOLFE = 2,     /* credit the member of smallest (rho, id) */
+  KCL_PAVA_PREPROCESS = 3  /* credit the member of largest level */
+} kcl_task;
+
+const char *kcl_last_error(void);
+const char *kcl_status_name(kcl_status s);
+
+/* Largest k the GPU enumerator accepts (the reference takes any
+ * unsigned char; k above the degeneracy + 1 has no cliques). */
+#define KCL_MAX_K 32
+
+/* ---------------------------------------------------------------------
+ * Edge lists.  Replaces ReadEdgeList (Graph.h:34, Graph.c:34-53): text,
+ * two unsigned integers per edge, read until the first token that does
+ * not parse; n_in = max id + 1.  *pairs is allocated by the library
+ * (2*m uint32, s0 t0 s1 t1 ...), released with kcl_free.  A path ending
+ * in ".u32" is read as raw little-endian uint32 pairs instead.
+ * ------------------------------------------------------------------- */
+kcl_status kcl_read_edge_list(const char *path, uint32_t **pairs, uint64_t *m,
+                              uint32_t *n_in);
+kcl_status kcl_write_edge_list(const char *path, const uint32_t *pairs, uint64_t m);
+void kcl_free(void *p);
+
+/* ---------------------------------------------------------------------
+ * Graph.  Replaces SortByCore + Relabel + MakeGraph (Graph.h:38-46,
+ * Graph.c:57-156): degeneracy ranks from the reference's heap order
+ * (host), relabel + orientation s > t + sorted DAG CSR (device).
+ * The input must be a simple graph (no self-loops or repeated pairs), as
+ * the reference assumes; self-loops are rejected, repeats are not checked.
+ * n_in = 0 means max id + 1.  Limits: m < 2^32.
+ * ------------------------------------------------------------------- */
+typedef struct kcl_graph kcl_graph;
+
+typedef struct kcl_graph_info {
+  uint32_t n_in; /* ids in the input (max id + 1) */
+  uint32_t n;    /* relabelled nodes: max rank over edge endpoints + 1 */
+  uint64_t m;    /* edges */
+  uint32_t core; /* max out-degree of the DAG ("core value", Graph.c:139) */
+} kcl_graph_info;
+
+kcl_status kcl_graph_create(const uint32_t *pairs, uint64_t m, uint32_t n_in, kcl_graph **out);
+void kcl_graph_destroy(kcl_graph *g); /* FreeGraph (Graph.h:28) */
+kcl_status kcl_graph_get_info(const kcl_graph *g, kcl_graph_info *info);
+/* rank[v] for v < n_in (Graph.c:82). */
+kcl_status kcl_graph_rank(const kcl_graph *g, uint32_t *rank);
+/* cd[n+1], adj[m]: out-neighbours ascending (Graph.c:131-145). */
+kcl_status kcl_graph_csr(const kcl_graph *g, uint32_t *cd, uint32_t *adj);
+/* st[2m]: edges sorted by (s, t) (Graph.c:125). */
+kcl_status kcl_graph_edges(const kcl_graph *g, uint32_t *st);
+/* CountEdges (Graph.h:51, Graph.c:158-166): edges with both ends in nodes[]. */
+kcl_status kcl_count_edges(const kcl_graph *g, uint32_t num_nodes, const uint32_t *nodes,
+                           uint32_t *m_out);
+
+/* ---------------------------------------------------------------------
+ * One enumeration pass with host vectors.  Replaces
+ * CDF_CliqueEnum(Graph *g, unsigned char k, task_t task)
+ * (kCDensestCmpSync.c:298-325) including the globals it reads and writes:
+ *   KCL_FRANK_WOLFE:     reads rho[n], acc[w] += 1.0 per clique
+ *                        (rho_add, kCDensestCmpSync.c:144-145)
+ *   KCL_PAVA_PREPROCESS: reads level[n], acc[level[w]] += 1.0
+ *                        (rho_tentative, kCDensestCmpSync.c:239-240)
+ * acc is accumulated into, like the reference's (callers zero it).
+ * visits (nullable) receives the number of k-cliques.  2 <= k <= KCL_MAX_K.
+ * ------------------------------------------------------------------- */
+kcl_status kcl_clique_enum(kcl_graph *g, uint32_t k, kcl_task task, const double *rho,
+                           const uint32_t *level, double *acc, uint64_t *visits);
+
+/* ---------------------------------------------------------------------
+ * Device-resident Sim-kClist++ state: rho (fp64), the per-iteration
+ * counts (uint64) and the exact totals V (uint64, rho_t = 2 V_t /
+ * ((t+1)(t+2))) stay in HBM across iterations.  Replaces AllocCdf +
+ * the loop body of CDF_Main (kCDensestCmpSync.c:130-136, 385-413).
+ * ------------------------------------------------------------------- */
+typedef struct kcl_sim kcl_sim;
+
+typedef enum kcl_schedule {
+  KCL_SCHED_DEGREE = 0, /* roots by estimated cost from degrees, heavy first */
+  KCL_SCHED_LEARNED = 1 /* roots ordered and split by the cost model (falls
+                           back to KCL_SCHED_DEGREE when none is loaded) */
+} kcl_schedule;
+
+typedef struct kcl_sim_opts {
+  uint32_t schedule;      /* kcl_schedule */
+  uint32_t split_work;    /* target work units per split root; 0 = default */
+  uint32_t reserved[6];
+} kcl_sim_opts;
+
+kcl_status kcl_sim_create(kcl_graph *g, uint32_t k, const kcl_sim_opts *opts, kcl_sim **out);
+void kcl_sim_destroy(kcl_sim *s);
+/* Iteration t >= 1: FW pass, then rho = (t*rho + 2*cnt)/(t + 2)
+ * (kCDensestCmpSync.c:399-403), fp64 in the reference's operation order. */
+kcl_status kcl_sim_step(kcl_sim *s, uint32_t t);
+/* Counts of the last pass (n uint64; rho_add of kCDensestCmpSync.c:401). */
+kcl_status kcl_sim_counts(kcl_sim *s, uint64_t *out);
+kcl_status kcl_sim_rho(kcl_sim *s, double *out);
+kcl_status kcl_sim_set_rho(kcl_sim *s, const double *rho);
+kcl_status kcl_sim_totals(kcl_sim *s, uint64_t *V);
+/* k-cliques visited by the last pass and its GPU time (HIP events on the
+ * pass's stream, enumeration kernels only / whole pass). */
+kcl_status kcl_sim_last_pass(kcl_sim *s, uint64_t *visits, double *kernel_ms, double *pass_ms);
+
+typedef struct kcl_densest_info {
+  uint32_t n;       /* nodes of the densest prefix */
+  uint32_t m;       /* edges inside it */
+  double density;   /* k-cliques / n as the reference computes it */
+  double ub;        /* the reference's upper bound */
+  uint64_t sum;     /* exact k-clique count of the prefix (density = sum/n) */
+} kcl_densest_info;
+
+/* CDF_FindDensestSubset (kCDensestCmpSync.c:342-383): level order by
+ * floor(rho) descending then id, PAVA-preprocess pass, densest prefix,
+ * CountEdges, upper bound.  reordered/level/tent (n each) are nullable. */
+kcl_status kcl_sim_find_densest(kcl_sim *s, kcl_densest_info *out, uint32_t *reordered,
+                                uint32_t *level, uint64_t *tent);
+
+/* ---------------------------------------------------------------------
+ * Whole Sim-kClist++ run: CDF_Main (kCDensestCmpSync.c:385-419).
+ * Iterations run up to the largest power of two <= num_iter, with one
+ * record per power of two.  stat_path (nullable) receives the reference's
+ * file: header + "%u\t%u\t%u\t%.12f\t%.12f\t%ld\n" (T, n, m, density, ub,
+ * whole seconds since t0_unix).  records (nullable) gets up to 32 rows.
+ * ------------------------------------------------------------------- */
+typedef struct kcl_record {
+  uint32_t T;
+  uint32_t n;
+  uint32_t m;
+  double density;
+  double ub;
+  uint64_t sum;
+  double seconds;
+} kcl_record;
+
+kcl_status kcl_cmpsync_run(kcl_graph *g, uint32_t k, uint32_t num_iter, const kcl_sim_opts *opts,
+                           const char *stat_path, int64_t t0_unix, kcl_record *records,
+                           uint32_t *nrec);
+
+/* ---------------------------------------------------------------------
+ * Library information.
+ * ------------------------------------------------------------------- */
+typedef struct kcl_device_info {
+  char name[64];
+  char arch[32];
+  uint32_t cu_count;
+  uint64_t hbm_bytes;
+  uint32_t lds_bytes_per_cu;
+} kcl_device_info;
+
+kcl_status kcl_device_get_info(kcl_device_info *info);
+const char *kcl_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KCLPP_H */

@@ -1,0 +1,352 @@
+"""MI355X-native Sim-kClist++ (k-clique densest subgraph) - Python host side.
+
+Thin ctypes layer over ``libkclpp.so`` (the C ABI in ``include/kclpp.h``).
+The names mirror the reference programs (btsun/kclistpp):
+
+* ``read_edge_list``  - ReadEdgeList (Graph.c:34-53)
+* ``Graph``           - SortByCore + Relabel + MakeGraph (Graph.c:57-156)
+* ``Graph.clique_enum`` - CDF_CliqueEnum (kCDensestCmpSync.c:298-325)
+* ``Sim``             - AllocCdf + the CDF_Main loop body + CDF_FindDensestSubset
+* ``cmpsync``         - CDF_Main (kCDensestCmpSync.c:385-419)
+
+The library is required: importing works without it, but every call raises
+``KclError`` if ``libkclpp.so`` is missing (no CPU fallback exists).
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libkclpp.so")
+
+FRANK_WOLFE = 2
+PAVA_PREPROCESS = 3
+SCHED_DEGREE = 0
+SCHED_LEARNED = 1
+
+_STATUS = {
+    0: "KCL_OK", 1: "KCL_E_IO", 2: "KCL_E_PARSE", 3: "KCL_E_INVALID_ARG",
+    4: "KCL_E_OOM_HOST", 5: "KCL_E_OOM_DEVICE", 6: "KCL_E_HIP", 7: "KCL_E_OVERFLOW",
+    8: "KCL_E_UNSUPPORTED",
+}
+
+
+class KclError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"{_STATUS.get(status, status)}: {message}")
+        self.status = status
+        self.name = _STATUS.get(status, str(status))
+
+
+class _GraphInfo(C.Structure):
+    _fields_ = [("n_in", C.c_uint32), ("n", C.c_uint32), ("m", C.c_uint64),
+                ("core", C.c_uint32)]
+
+
+class _SimOpts(C.Structure):
+    _fields_ = [("schedule", C.c_uint32), ("split_work", C.c_uint32),
+                ("reserved", C.c_uint32 * 6)]
+
+
+class _Densest(C.Structure):
+    _fields_ = [("n", C.c_uint32), ("m", C.c_uint32), ("density", C.c_double),
+                ("ub", C.c_double), ("sum", C.c_uint64)]
+
+
+class _Record(C.Structure):
+    _fields_ = [("T", C.c_uint32), ("n", C.c_uint32), ("m", C.c_uint32),
+                ("density", C.c_double), ("ub", C.c_double), ("sum", C.c_uint64),
+                ("seconds", C.c_double)]
+
+
+class _DeviceInfo(C.Structure):
+    _fields_ = [("name", C.c_char * 64), ("arch", C.c_char * 32), ("cu_count", C.c_uint32),
+                ("hbm_bytes", C.c_uint64), ("lds_bytes_per_cu", C.c_uint32)]
+
+
+_lib = None
+
+EXPORTS = [
+    "kcl_last_error", "kcl_status_name", "kcl_read_edge_list", "kcl_write_edge_list", "kcl_free",
+    "kcl_graph_create", "kcl_graph_destroy", "kcl_graph_get_info", "kcl_graph_rank",
+    "kcl_graph_csr", "kcl_graph_edges", "kcl_count_edges", "kcl_clique_enum", "kcl_sim_create",
+    "kcl_sim_destroy", "kcl_sim_step", "kcl_sim_counts", "kcl_sim_rho", "kcl_sim_set_rho",
+    "kcl_sim_totals", "kcl_sim_last_pass", "kcl_sim_find_densest", "kcl_cmpsync_run",
+    "kcl_device_get_info", "kcl_version",
+]
+
+
+def lib() -> C.CDLL:
+    """Load libkclpp.so (raises KclError if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise KclError(6, f"{LIB_PATH} is not built; run `python -m kclistpp_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    vp, u32, u64, i64 = C.c_void_p, C.c_uint32, C.c_uint64, C.c_int64
+    sigs = {
+        "kcl_last_error": ([], C.c_char_p),
+        "kcl_status_name": ([C.c_int], C.c_char_p),
+        "kcl_version": ([], C.c_char_p),
+        "kcl_read_edge_list": ([C.c_char_p, C.POINTER(C.POINTER(C.c_uint32)), C.POINTER(u64),
+                                C.POINTER(u32)], C.c_int),
+        "kcl_write_edge_list": ([C.c_char_p, vp, u64], C.c_int),
+        "kcl_free": ([vp], None),
+        "kcl_graph_create": ([vp, u64, u32, C.POINTER(vp)], C.c_int),
+        "kcl_graph_destroy": ([vp], None),
+        "kcl_graph_get_info": ([vp, C.POINTER(_GraphInfo)], C.c_int),
+        "kcl_graph_rank": ([vp, vp], C.c_int),
+        "kcl_graph_csr": ([vp, vp, vp], C.c_int),
+        "kcl_graph_edges": ([vp, vp], C.c_int),
+        "kcl_count_edges": ([vp, u32, vp, C.POINTER(u32)], C.c_int),
+        "kcl_clique_enum": ([vp, u32, C.c_int, vp, vp, vp, C.POINTER(u64)], C.c_int),
+        "kcl_sim_create": ([vp, u32, C.POINTER(_SimOpts), C.POINTER(vp)], C.c_int),
+        "kcl_sim_destroy": ([vp], None),
+        "kcl_sim_step": ([vp, u32], C.c_int),
+        "kcl_sim_counts": ([vp, vp], C.c_int),
+        "kcl_sim_rho": ([vp, vp], C.c_int),
+        "kcl_sim_set_rho": ([vp, vp], C.c_int),
+        "kcl_sim_totals": ([vp, vp], C.c_int),
+        "kcl_sim_last_pass": ([vp, C.POINTER(u64), C.POINTER(C.c_double),
+                               C.POINTER(C.c_double)], C.c_int),
+        "kcl_sim_find_densest": ([vp, C.POINTER(_Densest), vp, vp, vp], C.c_int),
+        "kcl_cmpsync_run": ([vp, u32, u32, C.POINTER(_SimOpts), C.c_char_p, i64,
+                             C.POINTER(_Record), C.POINTER(u32)], C.c_int),
+        "kcl_device_get_info": ([C.POINTER(_DeviceInfo)], C.c_int),
+    }
+    for name, (args, res) in sigs.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = L
+    return L
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        msg = lib().kcl_last_error().decode(errors="replace")
+        raise KclError(rc, msg)
+
+
+def _ptr(a: np.ndarray | None):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def version() -> str:
+    return lib().kcl_version().decode()
+
+
+def device_info() -> dict:
+    d = _DeviceInfo()
+    _check(lib().kcl_device_get_info(C.byref(d)))
+    return {"name": d.name.decode(), "arch": d.arch.decode(), "cu_count": d.cu_count,
+            "hbm_bytes": d.hbm_bytes, "lds_bytes_per_cu": d.lds_bytes_per_cu}
+
+
+def read_edge_list(path: str) -> tuple[np.ndarray, int]:
+    """ReadEdgeList: returns ((m, 2) uint32 pairs, n_in = max id + 1)."""
+    L = lib()
+    p = C.POINTER(C.c_uint32)()
+    m = C.c_uint64(0)
+    n = C.c_uint32(0)
+    _check(L.kcl_read_edge_list(path.encode(), C.byref(p), C.byref(m), C.byref(n)))
+    try:
+        if m.value:
+            arr = np.ctypeslib.as_array(p, shape=(m.value * 2,)).copy().reshape(-1, 2)
+        else:
+            arr = np.zeros((0, 2), np.uint32)
+    finally:
+        L.kcl_free(C.cast(p, C.c_void_p))
+    return arr, int(n.value)
+
+
+def write_edge_list(path: str, pairs: np.ndarray) -> None:
+    pairs = np.ascontiguousarray(pairs, dtype=np.uint32).reshape(-1, 2)
+    _check(lib().kcl_write_edge_list(path.encode(), _ptr(pairs), pairs.shape[0]))
+
+
+@dataclass
+class Densest:
+    n: int
+    m: int
+    density: float
+    ub: float
+    sum: int
+
+
+class Graph:
+    """Degeneracy-ordered DAG on the GPU (SortByCore + Relabel + MakeGraph)."""
+
+    def __init__(self, pairs: np.ndarray, n_in: int = 0):
+        pairs = np.ascontiguousarray(pairs, dtype=np.uint32).reshape(-1, 2)
+        h = C.c_void_p()
+        _check(lib().kcl_graph_create(_ptr(pairs) if pairs.size else None, pairs.shape[0],
+                                      n_in, C.byref(h)))
+        self._h = h
+        info = _GraphInfo()
+        _check(lib().kcl_graph_get_info(h, C.byref(info)))
+        self.n_in, self.n, self.m, self.core = info.n_in, info.n, int(info.m), info.core
+
+    @classmethod
+    def from_file(cls, path: str) -> "Graph":
+        pairs, n_in = read_edge_list(path)
+        return cls(pairs, n_in)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().kcl_graph_destroy(self._h)
+            self._h = None
+
+    def __del__(self):  # pragma: no cover - interpreter shutdown order
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def handle(self):
+        if not self._h:
+            raise KclError(3, "graph is closed")
+        return self._h
+
+    def rank(self) -> np.ndarray:
+        out = np.zeros(max(self.n_in, 1), np.uint32)
+        _check(lib().kcl_graph_rank(self.handle, _ptr(out)))
+        return out[:self.n_in]
+
+    def csr(self) -> tuple[np.ndarray, np.ndarray]:
+        cd = np.zeros(self.n + 1, np.uint32)
+        adj = np.zeros(max(self.m, 1), np.uint32)
+        _check(lib().kcl_graph_csr(self.handle, _ptr(cd), _ptr(adj)))
+        return cd, adj[:self.m]
+
+    def edges(self) -> np.ndarray:
+        st = np.zeros(max(2 * self.m, 2), np.uint32)
+        _check(lib().kcl_graph_edges(self.handle, _ptr(st)))
+        return st[:2 * self.m].reshape(-1, 2)
+
+    def count_edges(self, nodes: np.ndarray) -> int:
+        nodes = np.ascontiguousarray(nodes, dtype=np.uint32)
+        out = C.c_uint32(0)
+        _check(lib().kcl_count_edges(self.handle, nodes.size, _ptr(nodes) if nodes.size else None,
+                                     C.byref(out)))
+        return int(out.value)
+
+    def clique_enum(self, k: int, task: int, rho: np.ndarray | None = None,
+                    level: np.ndarray | None = None, acc: np.ndarray | None = None):
+        """CDF_CliqueEnum: returns (acc, visits); acc is accumulated into."""
+        if acc is None:
+            acc = np.zeros(max(self.n, 1), np.float64)
+        rho_a = None if rho is None else np.ascontiguousarray(rho, np.float64)
+        lev_a = None if level is None else np.ascontiguousarray(level, np.uint32)
+        v = C.c_uint64(0)
+        _check(lib().kcl_clique_enum(self.handle, k, task, _ptr(rho_a), _ptr(lev_a), _ptr(acc),
+                                     C.byref(v)))
+        return acc[:self.n], int(v.value)
+
+
+def _opts(schedule: int = SCHED_DEGREE, split_work: int = 0):
+    o = _SimOpts()
+    o.schedule = schedule
+    o.split_work = split_work
+    return o
+
+
+class Sim:
+    """Device-resident Sim-kClist++ state for one graph and k."""
+
+    def __init__(self, graph: Graph, k: int, schedule: int = SCHED_DEGREE, split_work: int = 0):
+        self.graph = graph
+        self.k = k
+        h = C.c_void_p()
+        o = _opts(schedule, split_work)
+        _check(lib().kcl_sim_create(graph.handle, k, C.byref(o), C.byref(h)))
+        self._h = h
+        self.t = 0
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().kcl_sim_destroy(self._h)
+            self._h = None
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def step(self, t: int | None = None) -> None:
+        """One iteration (FW pass + rho update); t defaults to the next one."""
+        t = self.t + 1 if t is None else t
+        _check(lib().kcl_sim_step(self._h, t))
+        self.t = t
+
+    def counts(self) -> np.ndarray:
+        out = np.zeros(max(self.graph.n, 1), np.uint64)
+        _check(lib().kcl_sim_counts(self._h, _ptr(out)))
+        return out[:self.graph.n]
+
+    def rho(self) -> np.ndarray:
+        out = np.zeros(max(self.graph.n, 1), np.float64)
+        _check(lib().kcl_sim_rho(self._h, _ptr(out)))
+        return out[:self.graph.n]
+
+    def set_rho(self, rho: np.ndarray) -> None:
+        rho = np.ascontiguousarray(rho, np.float64)
+        _check(lib().kcl_sim_set_rho(self._h, _ptr(rho)))
+
+    def totals(self) -> np.ndarray:
+        out = np.zeros(max(self.graph.n, 1), np.uint64)
+        _check(lib().kcl_sim_totals(self._h, _ptr(out)))
+        return out[:self.graph.n]
+
+    def last_pass(self) -> tuple[int, float, float]:
+        v = C.c_uint64(0)
+        km = C.c_double(0)
+        pm = C.c_double(0)
+        _check(lib().kcl_sim_last_pass(self._h, C.byref(v), C.byref(km), C.byref(pm)))
+        return int(v.value), float(km.value), float(pm.value)
+
+    def find_densest(self, want_vectors: bool = False):
+        d = _Densest()
+        n = max(self.graph.n, 1)
+        reo = np.zeros(n, np.uint32) if want_vectors else None
+        lev = np.zeros(n, np.uint32) if want_vectors else None
+        tent = np.zeros(n, np.uint64) if want_vectors else None
+        _check(lib().kcl_sim_find_densest(self._h, C.byref(d), _ptr(reo), _ptr(lev), _ptr(tent)))
+        info = Densest(d.n, d.m, d.density, d.ub, int(d.sum))
+        if want_vectors:
+            m = self.graph.n
+            return info, reo[:m], lev[:m], tent[:m]
+        return info
+
+
+def cmpsync(graph: Graph, k: int, num_iter: int, stat_path: str | None = None,
+            schedule: int = SCHED_DEGREE, t0_unix: int = 0) -> list[dict]:
+    """CDF_Main: one record per power of two <= num_iter."""
+    recs = (_Record * 32)()
+    nrec = C.c_uint32(0)
+    o = _opts(schedule)
+    _check(lib().kcl_cmpsync_run(graph.handle, k, num_iter, C.byref(o),
+                                 stat_path.encode() if stat_path else None, t0_unix, recs,
+                                 C.byref(nrec)))
+    return [{"T": r.T, "n": r.n, "m": r.m, "density": r.density, "ub": r.ub, "sum": r.sum,
+             "seconds": r.seconds} for r in recs[:nrec.value]]

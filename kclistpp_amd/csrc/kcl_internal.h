@@ -1,0 +1,129 @@
+// kcl_internal.h - internal declarations of libkclpp (not part of the ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "kclpp.h"
+
+namespace kcl {
+
+// Thread-local error message; returns s so callers can `return fail(...)`.
+kcl_status fail(kcl_status s, const char *fmt, ...);
+
+#define KCL_HIP(call)                                                               \
+  do {                                                                              \
+    hipError_t e_ = (call);                                                         \
+    if (e_ != hipSuccess) {                                                         \
+      return ::kcl::fail(e_ == hipErrorOutOfMemory ? KCL_E_OOM_DEVICE : KCL_E_HIP,  \
+                         "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_),     \
+                         __FILE__, __LINE__);                                       \
+    }                                                                               \
+  } while (0)
+
+#define KCL_TRY(expr)                       \
+  do {                                      \
+    kcl_status s_ = (expr);                 \
+    if (s_ != KCL_OK) return s_;            \
+  } while (0)
+
+// Degeneracy ranks by the reference's heap order (host, sequential).
+void degeneracy_rank(uint32_t n, uint64_t m, const uint32_t *pairs, uint32_t *rank);
+
+// Device buffer that frees itself.
+template <class T>
+struct DevBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  kcl_status alloc(size_t count) {
+    release();
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc(&p, count * sizeof(T));
+    if (e != hipSuccess) {
+      p = nullptr;
+      return fail(e == hipErrorOutOfMemory ? KCL_E_OOM_DEVICE : KCL_E_HIP,
+                  "hipMalloc(%zu bytes): %s", count * sizeof(T), hipGetErrorString(e));
+    }
+    n = count;
+    return KCL_OK;
+  }
+};
+
+// Word-class limits of the enumeration kernels: a root's out-neighbourhood
+// of d nodes is held as d rows of W 64-bit words, W in {1,2,4,8,16}.
+constexpr uint32_t kMaxWords = 16;
+constexpr uint32_t kMaxRootDegree = 64 * kMaxWords;
+
+// One root task: root node s, split part p of np (items [p*I/np, (p+1)*I/np)).
+struct Task {
+  uint32_t s;
+  uint32_t part_np;  // part | (np << 16)
+};
+
+enum PassMode : uint32_t {
+  kCreditWinner = 0,  // each clique credits its minimum-key member
+  kCreditRoot = 1,    // each clique credits its root (per-root clique counts)
+};
+
+struct PassPlan {
+  // Tasks grouped by word class, heaviest first inside each class.
+  std::vector<Task> tasks;
+  uint32_t class_begin[6] = {0, 0, 0, 0, 0, 0};  // classes W = 1,2,4,8,16
+  uint32_t class_count[5] = {0, 0, 0, 0, 0};
+};
+
+struct DevicePlan {
+  DevBuf<Task> tasks;
+  DevBuf<uint32_t> tickets;  // one queue head per class
+  uint32_t class_begin[6] = {0, 0, 0, 0, 0, 0};
+  uint32_t class_count[5] = {0, 0, 0, 0, 0};
+};
+
+struct DeviceProps {
+  int device = 0;
+  uint32_t cu_count = 0;
+  uint32_t lds_per_cu = 0;
+  std::string arch;
+};
+kcl_status device_props(DeviceProps *out);
+
+// Enumeration pass over a plan: cnt[x] += credits.  key[x] orders members
+// (smaller wins, ties to the smaller id).  All launches go to `stream`;
+// the ms of the enumeration kernels is recorded between ev0 and ev1.
+kcl_status launch_pass(const uint32_t *d_cd, const uint32_t *d_adj, const uint64_t *d_key,
+                       uint32_t k, PassMode mode, DevicePlan &plan, unsigned long long *d_cnt,
+                       hipStream_t stream);
+
+// Work-unit estimate per root from a root-credit pass of (k-1)-cliques.
+kcl_status build_plan(const std::vector<uint32_t> &cd, uint32_t k,
+                      const std::vector<uint64_t> *root_cost, uint32_t split_work,
+                      PassPlan *plan);
+kcl_status upload_plan(const PassPlan &plan, DevicePlan *dev);
+
+}  // namespace kcl
+
+// Opaque ABI types.
+struct kcl_graph {
+  uint32_t n_in = 0;
+  uint32_t n = 0;
+  uint64_t m = 0;
+  uint32_t core = 0;
+  std::vector<uint32_t> rank;  // n_in
+  std::vector<uint32_t> cd;    // n + 1 (host copy)
+  std::vector<uint32_t> adj;   // m (host copy)
+  kcl::DevBuf<uint32_t> d_cd;
+  kcl::DevBuf<uint32_t> d_adj;
+  hipStream_t stream = nullptr;
+};

@@ -1,0 +1,472 @@
+// pass.hip - the k-clique enumeration pass on gfx950.
+//
+// Replaces CDF_CliqueEnum + MakeSubgraph + CDF_CliqueEnumThread +
+// CDF_FrankWolfeUpdateRates (kCDensestCmpSync.c:71-117, 138-325).
+//
+// Formulation.  Every k-clique Q has a unique largest id s (its root);
+// Q \ {s} is a (k-1)-clique inside L = N+(s), the root's out-neighbours.
+// The reference credits Q to its member of smallest (key, id), where key
+// is rho (Frank-Wolfe) or the reversed level (PAVA).  Orient the graph
+// induced on L by that order (u -> v when (key_u, id_u) < (key_v, id_v));
+// then the winner among Q \ {s} is its source u, and
+//     winner(Q) = u if (key_u, id_u) < (key_s, s) else s.
+// Since every member of L has a smaller id than s, that test is
+// key_u <= key_s.  So the pass needs, for every u in L, the number of
+// (k-2)-cliques inside A'[u] (u's out-set in the oriented graph), and
+// credits it to u or to s.  The same count with all credit sent to s
+// gives per-root clique counts (the cost pre-pass and trimming).
+//
+// Kernel shape.  One wavefront per root task, persistent, taking tasks
+// from a global ticket (tasks are sorted heaviest first).  L (<= 64*W
+// nodes) is staged in LDS with the members' keys; the oriented rows A'
+// are W 64-bit words each, built from the members' adjacency slices
+// (coalesced reads, binary search in L, ds_or into the row of the
+// earlier endpoint).  Counting is a per-lane state machine over items:
+//   k = 2: item u, one clique {s, u}
+//   k = 3: item u, popcount(A'[u])
+//   k = 4: item u,       P = A'[u],          steps w in P: popcount(P & A'[w])
+//   k >= 5: item (u, v),  P = A'[u] & A'[v], steps w in P: cliques of size
+//           k-4 inside P & A'[w] (nested loops, compile-time depth)
+// A lane that finishes an item takes its next one at once, so lanes stay
+// busy until the root's items run out.  Credits gather in LDS (u64 per
+// member) and are flushed with one global atomicAdd per non-zero member,
+// plus one wave-reduced atomicAdd for the root.
+//
+// Heavy roots are split into parts: part p of np takes items
+// [p*I/np, (p+1)*I/np) and rebuilds A' (the rebuild is cheap next to the
+// counting it splits).
+#include "kcl_internal.h"
+
+namespace kcl {
+
+namespace {
+
+template <int W>
+struct Lds {
+  static constexpr int D = 64 * W;
+  unsigned long long rows[D * W];  // oriented rows A'
+  uint64_t key[D];                 // member keys
+  unsigned long long cred[D];   // credits gathered per member
+  uint32_t node[D];             // member ids, ascending
+  uint32_t beg[D];              // cd[member]
+  uint32_t pre[D + 1];          // prefix sums (member degrees, later |A'[u]|)
+};
+
+__device__ __forceinline__ uint32_t select_bit(unsigned long long x, uint32_t r) {
+  // Position of the r-th (0-based) set bit of x; r < popcount(x).
+  uint32_t pos = 0, c;
+  c = __popcll(x & 0xffffffffull);
+  if (r >= c) { r -= c; x >>= 32; pos += 32; }
+  c = __popc((uint32_t)x & 0xffffu);
+  if (r >= c) { r -= c; x >>= 16; pos += 16; }
+  c = __popc((uint32_t)x & 0xffu);
+  if (r >= c) { r -= c; x >>= 8; pos += 8; }
+  c = __popc((uint32_t)x & 0xfu);
+  if (r >= c) { r -= c; x >>= 4; pos += 4; }
+  c = __popc((uint32_t)x & 0x3u);
+  if (r >= c) { r -= c; x >>= 2; pos += 2; }
+  c = (uint32_t)x & 1u;
+  if (r >= c) pos += 1;
+  return pos;
+}
+
+template <int W>
+__device__ __forceinline__ uint32_t popc_set(const unsigned long long (&p)[W]) {
+  uint32_t c = 0;
+#pragma unroll
+  for (int i = 0; i < W; ++i) c += __popcll(p[i]);
+  return c;
+}
+
+// Number of J-cliques inside P (P is a set of members; cliques follow the
+// orientation, so each is counted once, from its source).
+template <int W, int J>
+__device__ __forceinline__ uint64_t cliques_in(const unsigned long long (&P)[W],
+                                               const unsigned long long *rows) {
+  if constexpr (J == 1) {
+    return popc_set<W>(P);
+  } else {
+    uint64_t total = 0;
+#pragma unroll
+    for (int wi = 0; wi < W; ++wi) {
+      uint64_t bits = P[wi];
+      while (bits) {
+        const uint32_t b = (uint32_t)__builtin_ctzll(bits);
+        bits &= bits - 1;
+        const unsigned long long *r = rows + (uint32_t)(wi * 64 + b) * W;
+        unsigned long long Q[W];
+#pragma unroll
+        for (int j = 0; j < W; ++j) Q[j] = P[j] & r[j];
+        if constexpr (J >= 3) {
+          if (popc_set<W>(Q) < (uint32_t)(J - 1)) continue;
+        }
+        total += cliques_in<W, J - 1>(Q, rows);
+      }
+    }
+    return total;
+  }
+}
+
+__device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+// In-place inclusive scan of a[0..d) by one wavefront.
+__device__ __forceinline__ void wave_scan(uint32_t *a, uint32_t d, uint32_t lane) {
+  const uint32_t chunk = (d + 63) >> 6;
+  const uint32_t lo = min(d, lane * chunk), hi = min(d, lo + chunk);
+  uint32_t sum = 0;
+  for (uint32_t i = lo; i < hi; ++i) sum += a[i];
+  uint32_t inc = sum;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    uint32_t v = __shfl_up(inc, off);
+    if (lane >= (uint32_t)off) inc += v;
+  }
+  uint32_t run = inc - sum;
+  for (uint32_t i = lo; i < hi; ++i) {
+    run += a[i];
+    a[i] = run;
+  }
+}
+
+template <int W, int K>
+__global__ __launch_bounds__(64) void pass_kernel(const uint32_t *__restrict__ cd,
+                                                  const uint32_t *__restrict__ adj,
+                                                  const uint64_t *__restrict__ key,
+                                                  const Task *__restrict__ tasks, uint32_t ntasks,
+                                                  uint32_t *__restrict__ ticket, uint32_t mode,
+                                                  unsigned long long *__restrict__ cnt) {
+  static_assert(K >= 2, "k >= 2");
+  __shared__ Lds<W> sh;
+  const uint32_t lane = threadIdx.x;
+  const bool to_root = (mode == kCreditRoot);
+
+  for (;;) {
+    uint32_t ti = 0;
+    if (lane == 0) ti = atomicAdd(ticket, 1u);
+    ti = __shfl(ti, 0);
+    if (ti >= ntasks) break;
+    const Task tk = tasks[ti];
+    const uint32_t s = tk.s;
+    const uint32_t part = tk.part_np & 0xffffu, np = tk.part_np >> 16;
+    const uint32_t b0 = cd[s];
+    const uint32_t d = cd[s + 1] - b0;
+    const uint64_t ks = key[s];
+
+    // Stage the members, their keys and adjacency extents.
+    for (uint32_t i = lane; i < d; i += 64) {
+      const uint32_t x = adj[b0 + i];
+      sh.node[i] = x;
+      sh.key[i] = key[x];
+      const uint32_t xb = cd[x];
+      sh.beg[i] = xb;
+      sh.pre[i + 1] = cd[x + 1] - xb;
+      sh.cred[i] = 0;
+    }
+    if (lane == 0) sh.pre[0] = 0;
+    if constexpr (K >= 3) {
+      for (uint32_t i = lane; i < d * W; i += 64) sh.rows[i] = 0;
+    }
+    __syncthreads();
+
+    unsigned long long root_acc = 0;
+    if constexpr (K == 2) {
+      // Cliques {s, u}: items are the members themselves.
+      const uint32_t lo = (uint32_t)((uint64_t)part * d / np);
+      const uint32_t hi = (uint32_t)((uint64_t)(part + 1) * d / np);
+      for (uint32_t u = lo + lane; u < hi; u += 64) {
+        if (!to_root && sh.key[u] <= ks) sh.cred[u] += 1;
+        else root_acc += 1;
+      }
+    } else {
+      // Oriented rows from the members' adjacency slices.
+      wave_scan(sh.pre + 1, d, lane);
+      __syncthreads();
+      const uint32_t total = sh.pre[d];
+      uint32_t cur = 0;
+      for (uint32_t e = lane; e < total; e += 64) {
+        uint32_t lo = cur, hi = d;  // member owning element e: pre[i] <= e < pre[i+1]
+        while (hi - lo > 1) {
+          const uint32_t mid = (lo + hi) >> 1;
+          if (sh.pre[mid] <= e) lo = mid;
+          else hi = mid;
+        }
+        cur = lo;
+        const uint32_t i = lo;
+        const uint32_t y = adj[sh.beg[i] + (e - sh.pre[i])];
+        uint32_t a = 0, b = i;  // y among members [0, i) (all smaller ids)
+        while (a < b) {
+          const uint32_t mid = (a + b) >> 1;
+          if (sh.node[mid] < y) a = mid + 1;
+          else b = mid;
+        }
+        if (a < i && sh.node[a] == y) {
+          const bool a_first = sh.key[a] <= sh.key[i];  // ties: smaller id (a) first
+          const uint32_t from = a_first ? a : i, to = a_first ? i : a;
+          atomicOr(&sh.rows[from * W + (to >> 6)], 1ull << (to & 63));
+        }
+      }
+      __syncthreads();
+
+      if constexpr (K == 3) {
+        const uint32_t lo = (uint32_t)((uint64_t)part * d / np);
+        const uint32_t hi = (uint32_t)((uint64_t)(part + 1) * d / np);
+        for (uint32_t u = lo + lane; u < hi; u += 64) {
+          uint32_t c = 0;
+#pragma unroll
+          for (int j = 0; j < W; ++j) c += __popcll(sh.rows[u * W + j]);
+          if (c) {
+            if (!to_root && sh.key[u] <= ks) sh.cred[u] += c;
+            else root_acc += c;
+          }
+        }
+      } else {
+        // Items: u (k = 4) or DAG edges (u, v) (k >= 5).
+        constexpr bool kEdgeItems = (K >= 5);
+        constexpr int kStepDepth = kEdgeItems ? K - 4 : 1;  // cliques counted per step
+        constexpr uint32_t kMinSet = kEdgeItems ? (uint32_t)(K - 3) : 2u;
+        uint32_t nitems = d;
+        if constexpr (kEdgeItems) {
+          for (uint32_t u = lane; u < d; u += 64) {
+            uint32_t c = 0;
+#pragma unroll
+            for (int j = 0; j < W; ++j) c += __popcll(sh.rows[u * W + j]);
+            sh.pre[u + 1] = c;
+          }
+          __syncthreads();
+          wave_scan(sh.pre + 1, d, lane);
+          __syncthreads();
+          nitems = sh.pre[d];
+        }
+        const uint32_t ilo = (uint32_t)((uint64_t)part * nitems / np);
+        const uint32_t ihi = (uint32_t)((uint64_t)(part + 1) * nitems / np);
+
+        uint32_t e = ilo + lane;
+        uint32_t u = 0, wi = 0;
+        unsigned long long P[W];
+        unsigned long long bits = 0;
+        unsigned long long acc = 0;
+        bool live = false;
+        uint32_t ucur = 0;
+
+        // Next item with |P| >= kMinSet, or live = false.
+        auto fetch = [&]() {
+          live = false;
+          while (e < ihi) {
+            const uint32_t it = e;
+            e += 64;
+            if constexpr (kEdgeItems) {
+              uint32_t lo = ucur, hi = d;  // pre[u] <= it < pre[u+1]
+              while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (sh.pre[mid] <= it) lo = mid;
+                else hi = mid;
+              }
+              ucur = lo;
+              u = lo;
+              uint32_t r = it - sh.pre[u];
+              uint32_t v = 0;
+              bool found = false;
+#pragma unroll
+              for (int j = 0; j < W; ++j) {
+                const unsigned long long word = sh.rows[u * W + j];
+                const uint32_t c = __popcll(word);
+                if (!found) {
+                  if (r < c) {
+                    v = (uint32_t)j * 64 + select_bit(word, r);
+                    found = true;
+                  } else {
+                    r -= c;
+                  }
+                }
+              }
+#pragma unroll
+              for (int j = 0; j < W; ++j) P[j] = sh.rows[u * W + j] & sh.rows[v * W + j];
+            } else {
+              u = it;
+#pragma unroll
+              for (int j = 0; j < W; ++j) P[j] = sh.rows[u * W + j];
+            }
+            if (popc_set<W>(P) >= kMinSet) {
+              wi = 0;
+              bits = P[0];
+              live = true;
+              return;
+            }
+          }
+        };
+
+        fetch();
+        while (live) {
+          if (bits) {
+            const uint32_t w = wi * 64 + (uint32_t)__builtin_ctzll(bits);
+            bits &= bits - 1;
+            const unsigned long long *r = sh.rows + w * W;
+            unsigned long long Q[W];
+#pragma unroll
+            for (int j = 0; j < W; ++j) Q[j] = P[j] & r[j];
+            acc += cliques_in<W, kStepDepth>(Q, sh.rows);
+          } else if (wi + 1 < (uint32_t)W) {
+            ++wi;
+            unsigned long long nb = 0;
+#pragma unroll
+            for (int j = 0; j < W; ++j) nb = (j == (int)wi) ? P[j] : nb;
+            bits = nb;
+          } else {
+            if (acc) {
+              if (!to_root && sh.key[u] <= ks) atomicAdd(&sh.cred[u], acc);
+              else root_acc += acc;
+              acc = 0;
+            }
+            fetch();
+          }
+        }
+      }
+    }
+
+    // Flush: the root's share (wave-reduced), then each member's.
+    root_acc = wave_sum(root_acc);
+    if (lane == 0 && root_acc) atomicAdd(&cnt[s], root_acc);
+    __syncthreads();
+    for (uint32_t i = lane; i < d; i += 64) {
+      const unsigned long long c = sh.cred[i];
+      if (c) atomicAdd(&cnt[sh.node[i]], c);
+    }
+    __syncthreads();
+  }
+}
+
+using PassFn = void (*)(const uint32_t *, const uint32_t *, const uint64_t *, const Task *,
+                        uint32_t, uint32_t *, uint32_t, unsigned long long *);
+
+constexpr int kMaxTemplK = 10;  // k handled by compiled kernels: 2..kMaxTemplK
+
+template <int W, int K>
+struct KTable {
+  static void fill(PassFn *t) {
+    t[K] = &pass_kernel<W, K>;
+    if constexpr (K < kMaxTemplK) KTable<W, K + 1>::fill(t);
+  }
+};
+
+struct PassTable {
+  PassFn fn[5][kMaxTemplK + 1] = {};
+  PassTable() {
+    KTable<1, 2>::fill(fn[0]);
+    KTable<2, 2>::fill(fn[1]);
+    KTable<4, 2>::fill(fn[2]);
+    KTable<8, 2>::fill(fn[3]);
+    KTable<16, 2>::fill(fn[4]);
+  }
+};
+
+PassFn kernel_for(uint32_t wclass, uint32_t k) {
+  static const PassTable table;  // thread-safe static initialisation
+  if (wclass > 4 || k < 2 || k > (uint32_t)kMaxTemplK) return nullptr;
+  return table.fn[wclass][k];
+}
+
+size_t lds_bytes(uint32_t wclass) {
+  switch (wclass) {
+    case 0: return sizeof(Lds<1>);
+    case 1: return sizeof(Lds<2>);
+    case 2: return sizeof(Lds<4>);
+    case 3: return sizeof(Lds<8>);
+    default: return sizeof(Lds<16>);
+  }
+}
+
+}  // namespace
+
+uint32_t max_pass_k() { return (uint32_t)kMaxTemplK; }
+
+kcl_status launch_pass(const uint32_t *d_cd, const uint32_t *d_adj, const uint64_t *d_key,
+                       uint32_t k, PassMode mode, DevicePlan &plan, unsigned long long *d_cnt,
+                       hipStream_t stream) {
+  DeviceProps props;
+  KCL_TRY(device_props(&props));
+  KCL_HIP(hipMemsetAsync(plan.tickets.p, 0, 8 * sizeof(uint32_t), stream));
+  for (int c = 4; c >= 0; --c) {  // heaviest word class first
+    const uint32_t count = plan.class_count[c];
+    if (!count) continue;
+    PassFn fn = kernel_for((uint32_t)c, k);
+    if (!fn) return fail(KCL_E_UNSUPPORTED, "no enumeration kernel for k = %u", k);
+    const size_t lds = lds_bytes((uint32_t)c);
+    uint32_t per_cu = (uint32_t)(props.lds_per_cu / (lds + 256));
+    if (per_cu < 1) per_cu = 1;
+    if (per_cu > 16) per_cu = 16;
+    uint64_t blocks = (uint64_t)props.cu_count * per_cu;
+    if (blocks > count) blocks = count;
+    hipLaunchKernelGGL(fn, dim3((uint32_t)blocks), dim3(64), 0, stream, d_cd, d_adj, d_key,
+                       plan.tasks.p + plan.class_begin[c], count, plan.tickets.p + c,
+                       (uint32_t)mode, d_cnt);
+    KCL_HIP(hipGetLastError());
+  }
+  return KCL_OK;
+}
+
+kcl_status build_plan(const std::vector<uint32_t> &cd, uint32_t k,
+                      const std::vector<uint64_t> *root_cost, uint32_t split_work,
+                      PassPlan *plan) {
+  const uint32_t n = cd.empty() ? 0 : (uint32_t)(cd.size() - 1);
+  struct Entry {
+    uint64_t cost;
+    uint32_t s;
+    uint32_t np;
+  };
+  std::vector<Entry> cls[5];
+  const uint32_t need = k - 1;  // members needed in N+(s)
+  for (uint32_t s = 0; s < n; ++s) {
+    const uint32_t d = cd[s + 1] - cd[s];
+    if (d < need || d == 0) continue;
+    if (d > kMaxRootDegree)
+      return fail(KCL_E_UNSUPPORTED,
+                  "node %u has out-degree %u in the DAG; the enumerator handles up to %u", s, d,
+                  kMaxRootDegree);
+    const uint32_t words = (d + 63) / 64;
+    const uint32_t c = words <= 1 ? 0 : words <= 2 ? 1 : words <= 4 ? 2 : words <= 8 ? 3 : 4;
+    uint64_t cost = root_cost ? (*root_cost)[s] : (uint64_t)d * d;
+    uint32_t np = 1;
+    if (split_work && cost > split_work) {
+      uint64_t want = (cost + split_work - 1) / split_work;
+      const uint64_t items = (k >= 5) ? (uint64_t)d * (d - 1) / 2 : d;
+      if (want > items) want = items;
+      if (want > 4096) want = 4096;
+      np = (uint32_t)(want ? want : 1);
+    }
+    cls[c].push_back(Entry{cost, s, np});
+  }
+  plan->tasks.clear();
+  uint32_t at = 0;
+  for (int c = 0; c < 5; ++c) {
+    std::stable_sort(cls[c].begin(), cls[c].end(),
+                     [](const Entry &a, const Entry &b) { return a.cost > b.cost; });
+    plan->class_begin[c] = at;
+    uint32_t cnt = 0;
+    for (const Entry &e : cls[c])
+      for (uint32_t p = 0; p < e.np; ++p) {
+        plan->tasks.push_back(Task{e.s, p | (e.np << 16)});
+        ++cnt;
+      }
+    plan->class_count[c] = cnt;
+    at += cnt;
+  }
+  plan->class_begin[5] = at;
+  return KCL_OK;
+}
+
+kcl_status upload_plan(const PassPlan &plan, DevicePlan *dev) {
+  KCL_TRY(dev->tasks.alloc(plan.tasks.size()));
+  KCL_TRY(dev->tickets.alloc(8));
+  if (!plan.tasks.empty())
+    KCL_HIP(hipMemcpy(dev->tasks.p, plan.tasks.data(), plan.tasks.size() * sizeof(Task),
+                      hipMemcpyHostToDevice));
+  for (int c = 0; c < 6; ++c) dev->class_begin[c] = plan.class_begin[c];
+  for (int c = 0; c < 5; ++c) dev->class_count[c] = plan.class_count[c];
+  return KCL_OK;
+}
+
+}  // namespace kcl

@@ -1,0 +1,161 @@
+/*
+ * ref_harness.c - drives the reference's own Sim-kClist++ code, compiled
+ * from its sources (kCDensestCmpSync.c, Graph.c, BinaryHeap.c) into
+ * oracle/_ref/libkclref.so by oracle/Makefile.
+ *
+ * TEST INFRASTRUCTURE ONLY: used to write golden vectors in the build
+ * container and as the timed CPU baseline.  It calls the reference's
+ * non-static symbols and reads its extern globals; the per-iteration loop
+ * below is kCDensestCmpSync.c:392-413 with the vectors captured between
+ * steps.
+ */
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+#include <omp.h>
+
+#include "Graph.h"
+
+extern double *rho;
+extern double *rho_add;
+extern double *rho_tentative;
+extern unsigned *level;
+extern unsigned *reordered;
+
+/* kCDensestCmpSync.c:128 and :327-332 */
+enum { REF_FRANK_WOLFE = 2, REF_PAVA_PREPROCESS = 3 };
+typedef struct {
+  unsigned n;
+  unsigned m;
+  double density;
+  double ub;
+} ref_info;
+
+void AllocCdf(Graph *g, unsigned k);
+void CDF_CliqueEnum(Graph *g, unsigned char k, int task);
+ref_info CDF_FindDensestSubset(Graph *g, unsigned char k);
+
+typedef struct {
+  Graph *g;
+  unsigned n_in;
+  unsigned *rank;
+  unsigned k;
+} kclref;
+
+/* ReadEdgeList -> SortByCore -> Relabel -> MakeGraph (kCDensestCmpSync.c:433-443),
+ * keeping a copy of the rank vector that MakeGraph frees. */
+void *kclref_load(const char *path) {
+  EdgeList *el = ReadEdgeList((char *)path);
+  kclref *h = (kclref *)calloc(1, sizeof(kclref));
+  h->n_in = el->n;
+  SortByCore(el);
+  h->rank = (unsigned *)malloc(el->n * sizeof(unsigned));
+  memcpy(h->rank, el->rank, el->n * sizeof(unsigned));
+  Relabel(el);
+  h->g = MakeGraph(el);
+  return h;
+}
+
+unsigned kclref_n_in(void *p) { return ((kclref *)p)->n_in; }
+unsigned kclref_n(void *p) { return ((kclref *)p)->g->n; }
+unsigned kclref_e(void *p) { return ((kclref *)p)->g->e; }
+unsigned kclref_core(void *p) { return ((kclref *)p)->g->core; }
+
+void kclref_rank(void *p, unsigned *out) {
+  kclref *h = (kclref *)p;
+  memcpy(out, h->rank, h->n_in * sizeof(unsigned));
+}
+
+void kclref_edges(void *p, unsigned *out) {
+  kclref *h = (kclref *)p;
+  for (unsigned i = 0; i < h->g->e; ++i) {
+    out[2 * i] = h->g->edges[i].s;
+    out[2 * i + 1] = h->g->edges[i].t;
+  }
+}
+
+void kclref_csr(void *p, unsigned *cd, unsigned *adj) {
+  kclref *h = (kclref *)p;
+  memcpy(cd, h->g->cd, (h->g->n + 1) * sizeof(unsigned));
+  memcpy(adj, h->g->adj, h->g->e * sizeof(unsigned));
+}
+
+void kclref_threads(int p) { omp_set_num_threads(p); }
+
+/* AllocCdf + rho = 0 (kCDensestCmpSync.c:388-393). */
+void kclref_start(void *p, unsigned k) {
+  kclref *h = (kclref *)p;
+  h->k = k;
+  AllocCdf(h->g, k);
+  for (unsigned i = 0; i < h->g->n; ++i) rho[i] = 0;
+}
+
+/* One Frank-Wolfe step t (kCDensestCmpSync.c:399-403); copies the pass's
+ * rho_add (before the update) to add_out and the updated rho to rho_out. */
+void kclref_step(void *p, unsigned t, double *add_out, double *rho_out) {
+  kclref *h = (kclref *)p;
+  Graph *g = h->g;
+  for (unsigned i = 0; i < g->n; ++i) rho_add[i] = 0;
+  CDF_CliqueEnum(g, (unsigned char)h->k, REF_FRANK_WOLFE);
+  if (add_out) memcpy(add_out, rho_add, g->n * sizeof(double));
+  for (unsigned i = 0; i < g->n; ++i) rho[i] = (t * rho[i] + 2.0 * rho_add[i]) / (t + 2.0);
+  if (rho_out) memcpy(rho_out, rho, g->n * sizeof(double));
+}
+
+/* Report step (kCDensestCmpSync.c:406-408): returns n, m, density, ub and
+ * copies level, reordered and rho_tentative. */
+void kclref_report(void *p, unsigned *nm, double *dens_ub, unsigned *level_out,
+                   unsigned *reordered_out, double *tent_out) {
+  kclref *h = (kclref *)p;
+  Graph *g = h->g;
+  for (unsigned i = 0; i < g->n; ++i) rho_tentative[i] = 0;
+  ref_info info = CDF_FindDensestSubset(g, (unsigned char)h->k);
+  nm[0] = info.n;
+  nm[1] = info.m;
+  dens_ub[0] = info.density;
+  dens_ub[1] = info.ub;
+  if (level_out) memcpy(level_out, level, g->n * sizeof(unsigned));
+  if (reordered_out) memcpy(reordered_out, reordered, g->n * sizeof(unsigned));
+  if (tent_out) memcpy(tent_out, rho_tentative, g->n * sizeof(double));
+}
+
+/* Set rho from the caller (to time or check a pass at a given state). */
+void kclref_set_rho(void *p, const double *src) {
+  kclref *h = (kclref *)p;
+  memcpy(rho, src, h->g->n * sizeof(double));
+}
+
+/* Times one CDF_CliqueEnum(FRANK_WOLFE) over a subset of the root edges
+ * (indices into the sorted edge array), with the current rho.  Returns the
+ * wall seconds; *visits gets the k-cliques credited (sum of rho_add). */
+double kclref_time_fw_roots(void *p, const unsigned *roots, unsigned nroots, double *visits) {
+  kclref *h = (kclref *)p;
+  Graph *g = h->g;
+  Edge *all = g->edges;
+  unsigned e_all = g->e;
+  Edge *sub = (Edge *)malloc((nroots ? nroots : 1) * sizeof(Edge));
+  for (unsigned i = 0; i < nroots; ++i) sub[i] = all[roots[i]];
+  for (unsigned i = 0; i < g->n; ++i) rho_add[i] = 0;
+  g->edges = sub;
+  g->e = nroots;
+  struct timespec a, b;
+  clock_gettime(CLOCK_MONOTONIC, &a);
+  CDF_CliqueEnum(g, (unsigned char)h->k, REF_FRANK_WOLFE);
+  clock_gettime(CLOCK_MONOTONIC, &b);
+  g->edges = all;
+  g->e = e_all;
+  free(sub);
+  double s = 0;
+  for (unsigned i = 0; i < g->n; ++i) s += rho_add[i];
+  *visits = s;
+  return (double)(b.tv_sec - a.tv_sec) + 1e-9 * (double)(b.tv_nsec - a.tv_nsec);
+}
+
+void kclref_free(void *p) {
+  kclref *h = (kclref *)p;
+  FreeGraph(h->g);
+  free(h->rank);
+  free(h);
+}

@@ -1,0 +1,226 @@
+"""GPU parity of the Sim-kClist++ path against the reference's golden vectors
+and the oracle.  Bit-exact: ranks, DAG, per-iteration uint64 counts, rho
+bits, level/tent, and the report records (n, m, density bits, ub bits)."""
+
+import hashlib
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, golden_index, load_golden
+from kclistpp_amd import graphgen as gg
+
+pytestmark = pytest.mark.gpu
+
+SIM = golden_index()["sim"]
+BIG = golden_index()["big"]
+
+
+def _bits(x):
+    return int(np.float64(x).view(np.uint64))
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+@pytest.mark.parametrize("case", SIM)
+def test_graph_build_matches_reference(kcl, case):
+    gd = load_golden(case)
+    _say("graph", case)
+    with kcl.Graph(gd["pairs"]) as g:
+        assert (g.n, g.core) == (int(gd["n"]), int(gd["core"]))
+        assert np.array_equal(g.rank(), gd["rank"])
+        e = g.edges()
+        assert np.array_equal(e[:, 0], gd["es"]) and np.array_equal(e[:, 1], gd["et"])
+        cd, adj = g.csr()
+        assert cd[-1] == g.m and np.array_equal(adj, gd["et"])
+
+
+@pytest.mark.parametrize("case", SIM)
+def test_sim_iterations_match_reference(kcl, case):
+    gd = load_golden(case)
+    k, T = int(gd["k"]), int(gd["num_iter"])
+    _say("sim", case)
+    with kcl.Graph(gd["pairs"]) as g, kcl.Sim(g, k) as s:
+        recs = []
+        t = 1
+        Tp = 1
+        while Tp <= T:
+            while t <= Tp:
+                s.step(t)
+                c = s.counts()
+                assert np.array_equal(c, gd["counts"][t - 1]), f"counts differ at t={t}"
+                visits, _, _ = s.last_pass()
+                assert visits == int(gd["counts"][t - 1].sum())
+                t += 1
+            info, reo, level, tent = s.find_densest(want_vectors=True)
+            recs.append([Tp, info.n, info.m, _bits(info.density), _bits(info.ub)])
+            Tp <<= 1
+        assert np.array_equal(s.rho().view(np.uint64), gd["rho_bits"])
+        assert np.array_equal(np.array(recs, np.uint64), gd["records"])
+        assert np.array_equal(level, gd["level"])
+        assert np.array_equal(tent, gd["tent"])
+        # Exact totals: rho_t = 2 V_t / ((t+1)(t+2)).
+        V = s.totals()
+        tt = t - 1
+        assert np.array_equal(V, sum((i + 1) * gd["counts"][i - 1].astype(np.uint64)
+                                     for i in range(1, tt + 1)))
+
+
+@pytest.mark.parametrize("case", [c for c in SIM if "cl3k" in c or "k7" in c])
+def test_cmpsync_driver_matches_reference(kcl, case, tmp_path):
+    gd = load_golden(case)
+    k, T = int(gd["k"]), int(gd["num_iter"])
+    _say("cmpsync", case)
+    stat = str(tmp_path / "stat.txt")
+    with kcl.Graph(gd["pairs"]) as g:
+        recs = kcl.cmpsync(g, k, T, stat_path=stat)
+    got = np.array([[r["T"], r["n"], r["m"], _bits(r["density"]), _bits(r["ub"])] for r in recs],
+                   np.uint64)
+    assert np.array_equal(got, gd["records"])
+    lines = open(stat).read().splitlines()
+    assert lines[0].startswith("[Number of Iterations]")
+    for line, r in zip(lines[1:], gd["records"]):
+        f = line.split("\t")
+        want = "%u\t%u\t%u\t%.12f\t%.12f" % (r[0], r[1], r[2], np.uint64(r[3]).view(np.float64),
+                                            np.uint64(r[4]).view(np.float64))
+        assert "\t".join(f[:5]) == want
+
+
+CLI = os.path.join(ROOT, "kclistpp_amd", "bin", "kCDensestCmpSync")
+
+
+@pytest.mark.skipif(not os.path.exists(CLI), reason="CLI not built")
+def test_cli_drop_in_stat_file(kcl, tmp_path):
+    gd = load_golden("sim_cl3k_k4_T8")
+    path = str(tmp_path / "g.txt")
+    gg.write_text(gd["pairs"], path)
+    _say("cli cl3k k=4 T=10")
+    r = subprocess.run([CLI, "8", "10", "4", path, "cl3k"], cwd=tmp_path, capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    lines = open(tmp_path / "stat_cmpsync_cl3k_8_10_4.txt").read().splitlines()
+    assert len(lines) == 1 + len(gd["records"])
+    for line, rec in zip(lines[1:], gd["records"]):
+        f = line.split("\t")
+        assert [int(f[0]), int(f[1]), int(f[2])] == rec[:3].tolist()
+        assert f[3] == "%.12f" % np.uint64(rec[3]).view(np.float64)
+        assert f[4] == "%.12f" % np.uint64(rec[4]).view(np.float64)
+
+
+@pytest.mark.parametrize("case", BIG)
+def test_full_size_workloads_match_reference(kcl, case):
+    gd = load_golden(case)
+    pairs = gg.workload(str(gd["workload"][0]), seed=int(gd["seed"]),
+                        scale_down=int(gd["scale_down"]))
+    assert hashlib.sha256(pairs.tobytes()).hexdigest() == str(gd["pairs_sha"][0])
+    k, T = int(gd["k"]), int(gd["num_iter"])
+    _say("big", case)
+    with kcl.Graph(pairs) as g, kcl.Sim(g, k) as s:
+        assert hashlib.sha256(g.rank().tobytes()).hexdigest() == str(gd["rank_sha"][0])
+        assert hashlib.sha256(g.edges().tobytes()).hexdigest() == str(gd["edges_sha"][0])
+        shas, recs = [], []
+        t, Tp = 1, 1
+        while Tp <= T:
+            while t <= Tp:
+                s.step(t)
+                shas.append(hashlib.sha256(s.counts().tobytes()).hexdigest())
+                t += 1
+            d = s.find_densest()
+            recs.append([Tp, d.n, d.m, _bits(d.density), _bits(d.ub)])
+            Tp <<= 1
+        assert shas == [str(x) for x in gd["count_sha"]]
+        assert np.array_equal(np.array(recs, np.uint64), gd["records"])
+        assert hashlib.sha256(s.rho().view(np.uint64).tobytes()).hexdigest() == \
+            str(gd["rho_sha"][0])
+
+
+# ---------------------------------------------------------------------------
+# Against the oracle: the in-process boundary with arbitrary rho / level.
+
+def _community_graph(seed, n=2000, m=12000, size=40, p=0.8):
+    pairs, rng = gg.chung_lu(n, m, 2.3, 200.0, seed)
+    pairs = gg.plant_community(pairs, n, size, p, rng)
+    return gg.finalize(pairs, rng)
+
+
+@pytest.mark.parametrize("k", list(range(2, 11)))
+def test_clique_enum_boundary_vs_oracle(kcl, oracle, k):
+    pairs = _community_graph(100 + k)
+    og = oracle.dag(pairs)
+    rng = np.random.default_rng(k)
+    rho = rng.integers(0, 4, og.n).astype(np.float64) * 0.25  # many ties
+    _say("clique_enum FW k", k)
+    with kcl.Graph(pairs) as g:
+        acc, visits = g.clique_enum(k, kcl.FRANK_WOLFE, rho=rho)
+        want, wv = oracle.fw_counts(og, k, rho)
+        assert visits == wv
+        assert np.array_equal(acc, want.astype(np.float64))
+        level = rng.permutation(og.n).astype(np.uint32)
+        _say("clique_enum PAVA k", k)
+        acc, _ = g.clique_enum(k, kcl.PAVA_PREPROCESS, level=level)
+        assert np.array_equal(acc, oracle.pava_counts(og, k, level).astype(np.float64))
+
+
+def _bipartite_plus(part, p_in, seed):
+    """Complete bipartite K_{part,part} plus sparse edges inside each side:
+    out-degrees reach ~part (the widest word classes) with few cliques."""
+    rng = np.random.default_rng(seed)
+    a = np.arange(part)
+    b = np.arange(part, 2 * part)
+    cross = np.stack(np.meshgrid(a, b, indexing="ij"), -1).reshape(-1, 2)
+    intra = []
+    for side in (a, b):
+        iu, ju = np.triu_indices(part, 1)
+        keep = rng.random(iu.size) < p_in
+        intra.append(np.stack([side[iu[keep]], side[ju[keep]]], 1))
+    pairs = np.concatenate([cross, *intra]).astype(np.uint32)
+    return gg.finalize(pairs, rng)
+
+
+@pytest.mark.parametrize("part,k", [(300, 3), (300, 4), (300, 5), (560, 3), (560, 4), (560, 5)])
+def test_wide_neighbourhoods_vs_oracle(kcl, oracle, part, k):
+    pairs = _bipartite_plus(part, 0.02, part + k)
+    og = oracle.dag(pairs)
+    assert og.core > (256 if part == 300 else 512)
+    rng = np.random.default_rng(7)
+    rho = rng.integers(0, 3, og.n).astype(np.float64)
+    _say("wide", part, k, "core", og.core)
+    with kcl.Graph(pairs) as g:
+        acc, visits = g.clique_enum(k, kcl.FRANK_WOLFE, rho=rho)
+        want, wv = oracle.fw_counts(og, k, rho)
+        assert visits == wv and np.array_equal(acc, want.astype(np.float64))
+
+
+@pytest.mark.parametrize("split", [1, 7, 100])
+def test_counts_do_not_depend_on_splitting(kcl, split):
+    pairs = _community_graph(5, size=60, p=0.9)
+    _say("split", split)
+    with kcl.Graph(pairs) as g:
+        base = kcl.Sim(g, 5)
+        other = kcl.Sim(g, 5, split_work=split)
+        for t in (1, 2, 3):
+            base.step(t)
+            other.step(t)
+            assert np.array_equal(base.counts(), other.counts())
+            assert np.array_equal(base.rho().view(np.uint64), other.rho().view(np.uint64))
+        base.close()
+        other.close()
+
+
+def test_trivial_and_empty_graphs(kcl):
+    k5 = np.array([(i, j) for i in range(5) for j in range(i + 1, 5)], np.uint32)
+    with kcl.Graph(k5) as g, kcl.Sim(g, 7) as s:  # k > core + 1: no cliques
+        s.step(1)
+        assert s.counts().sum() == 0
+        d = s.find_densest()
+        assert d.n == 1 and d.density == 0.0
+    with kcl.Graph(np.zeros((0, 2), np.uint32)) as g:
+        assert g.n == 0 and g.m == 0
+        assert kcl.cmpsync(g, 3, 4) and True
+    with pytest.raises(kcl.KclError) as e:
+        kcl.Graph(np.array([[3, 3]], np.uint32))
+    assert e.value.name == "KCL_E_INVALID_ARG"

@@ -1,0 +1,127 @@
+"""CPU-side checks of the product library: exports, edge-list reader and
+writer, CLI argument handling, workload generators.  No GPU calls."""
+
+import hashlib
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+
+
+def _declared_functions():
+    names = []
+    for h in sorted(os.listdir(os.path.join(ROOT, "include"))):
+        if not h.endswith(".h"):
+            continue
+        text = open(os.path.join(ROOT, "include", h)).read()
+        names += re.findall(r"^\s*(?:const\s+)?[\w\s\*]+?\b(kcl_\w+)\s*\(", text, re.M)
+    return sorted(set(names))
+
+
+def test_library_exports_every_declared_symbol(kcl):
+    declared = _declared_functions()
+    assert len(declared) >= 20
+    out = subprocess.run(["nm", "-D", "--defined-only", kcl.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    exported = set(re.findall(r" T (kcl_\w+)", out))
+    missing = [n for n in declared if n not in exported]
+    assert not missing, missing
+    for name in declared:  # resolvable through ctypes as well
+        getattr(kcl.lib(), name)
+
+
+def test_python_wrapper_binds_declared_symbols(kcl):
+    assert set(kcl.EXPORTS) <= set(_declared_functions())
+
+
+def test_reader_text_semantics(kcl, tmp_path):
+    p = tmp_path / "g.txt"
+    p.write_text("0 1\n  2\t3\n\n4 5 6 7\n8 x 9 10\n")
+    pairs, n_in = kcl.read_edge_list(str(p))
+    # fscanf("%u %u") stops at the first token that does not parse
+    assert pairs.tolist() == [[0, 1], [2, 3], [4, 5], [6, 7]]
+    assert n_in == 8
+
+
+def test_reader_signs_wrap_like_scanf(kcl, tmp_path):
+    p = tmp_path / "g.txt"
+    p.write_text("+1 2\n-5 3\n")
+    pairs, n_in = kcl.read_edge_list(str(p))
+    assert pairs.tolist() == [[1, 2], [2**32 - 5, 3]]
+    assert n_in == 2**32 - 4
+
+
+def test_reader_rejects_max_id(kcl, tmp_path):
+    p = tmp_path / "g.txt"
+    p.write_text("0 4294967295\n")
+    with pytest.raises(kcl.KclError) as e:
+        kcl.read_edge_list(str(p))
+    assert e.value.name == "KCL_E_UNSUPPORTED"
+
+
+def test_reader_empty_and_missing(kcl, tmp_path):
+    p = tmp_path / "empty.txt"
+    p.write_text("")
+    pairs, n_in = kcl.read_edge_list(str(p))
+    assert pairs.shape == (0, 2) and n_in == 1  # the reference's n = 0 + 1
+    with pytest.raises(kcl.KclError) as e:
+        kcl.read_edge_list(str(tmp_path / "nope.txt"))
+    assert e.value.name == "KCL_E_IO"
+
+
+def test_writer_round_trip(kcl, tmp_path):
+    rng = np.random.default_rng(0)
+    pairs = rng.integers(0, 10_000, (5000, 2)).astype(np.uint32)
+    for name in ("a.txt", "a.u32"):
+        path = str(tmp_path / name)
+        kcl.write_edge_list(path, pairs)
+        back, n_in = kcl.read_edge_list(path)
+        assert np.array_equal(back, pairs) and n_in == int(pairs.max()) + 1
+    bad = tmp_path / "bad.u32"
+    bad.write_bytes(b"\x00" * 12)
+    with pytest.raises(kcl.KclError) as e:
+        kcl.read_edge_list(str(bad))
+    assert e.value.name == "KCL_E_PARSE"
+
+
+def test_reader_matches_oracle_reader(kcl, oracle, tmp_path):
+    from kclistpp_amd import graphgen as gg
+    pairs = gg.workload("er20k", seed=3, scale_down=4)
+    path = str(tmp_path / "g.txt")
+    gg.write_text(pairs, path)
+    got, n_in = kcl.read_edge_list(path)
+    assert np.array_equal(got, pairs)
+    assert n_in == int(pairs.max()) + 1
+
+
+CLI = os.path.join(ROOT, "kclistpp_amd", "bin", "kCDensestCmpSync")
+
+
+@pytest.mark.skipif(not os.path.exists(CLI), reason="CLI not built")
+@pytest.mark.parametrize("args,code", [
+    ([], 2), (["8", "10", "4", "x"], 2), (["8", "ten", "4", "f", "t"], 2),
+    (["8", "10", "1", "f", "t"], 2), (["8", "10", "256", "f", "t"], 2),
+    (["8", "10", "4", "/nonexistent/file.txt", "t"], 1),
+])
+def test_cli_argument_errors(args, code, tmp_path):
+    r = subprocess.run([CLI, *args], capture_output=True, text=True, cwd=tmp_path)
+    assert r.returncode == code, r.stderr
+
+
+def test_generators_are_deterministic():
+    from kclistpp_amd import graphgen as gg
+    a = gg.workload("er20k", seed=1)
+    b = gg.workload("er20k", seed=1)
+    assert np.array_equal(a, b)
+    assert a.shape == (100435, 2)
+    # no loops, no repeats in either orientation
+    assert (a[:, 0] != a[:, 1]).all()
+    key = np.minimum(a[:, 0], a[:, 1]).astype(np.uint64) * 20000 + np.maximum(a[:, 0], a[:, 1])
+    assert np.unique(key).size == a.shape[0]
+    c = gg.workload("cl1m", seed=2, scale_down=256)
+    assert hashlib.sha256(c.tobytes()).hexdigest() == \
+        hashlib.sha256(gg.workload("cl1m", seed=2, scale_down=256).tobytes()).hexdigest()
