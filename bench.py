@@ -144,7 +144,8 @@ def main():
     sched = kcl.SCHED_LEARNED if args.schedule == "learned" else kcl.SCHED_DEGREE
     t0 = time.time()
     sim = kcl.Sim(g, k, schedule=sched)
-    log(f"bench: plan ready in {time.time() - t0:.1f}s; warmup {args.warmup}")
+    plan = sim.plan_info()
+    log(f"bench: plan ready in {time.time() - t0:.1f}s: {plan}; warmup {args.warmup}")
     t = 0
     for _ in range(args.warmup):
         t += 1
@@ -157,6 +158,8 @@ def main():
         t += 1
         sim.step(t)  # synchronous: returns after the pass and update completed
         v, kms, pms = sim.last_pass()
+        if v != plan["cliques"]:
+            raise RuntimeError(f"step {t} visited {v} cliques, the graph has {plan['cliques']}")
         visits += v
         kern_ms += kms
     elapsed = time.perf_counter() - t_start
@@ -190,7 +193,8 @@ def main():
         "data": "synthetic (seeded generator, kclistpp_amd/graphgen.py)",
         "config": {"workload": f"{args.workload}: {desc}", "k": k, "n": g.n, "m": g.m,
                    "core": g.core, "roots": nroots, "visits_per_step": visits // max(args.steps, 1),
-                   "schedule": args.schedule, "parallelism": "1 GPU"},
+                   "schedule": args.schedule, "parallelism": "1 GPU",
+                   "plan": plan},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -50,9 +50,11 @@ typedef enum kcl_task {
 const char *kcl_last_error(void);
 const char *kcl_status_name(kcl_status s);
 
-/* Largest k the GPU enumerator accepts (the reference takes any
- * unsigned char; k above the degeneracy + 1 has no cliques). */
-#define KCL_MAX_K 32
+/* Largest k the GPU enumerator is compiled for.  The reference takes any
+ * k in 2..255; here k above KCL_MAX_K is accepted when the graph cannot
+ * hold a k-clique (k > core + 1: every count is zero) and otherwise
+ * returns KCL_E_UNSUPPORTED. */
+#define KCL_MAX_K 10
 
 /* ---------------------------------------------------------------------
  * Edge lists.  Replaces ReadEdgeList (Graph.h:34, Graph.c:34-53): text,
@@ -105,7 +107,7 @@ kcl_status kcl_count_edges(const kcl_graph *g, uint32_t num_nodes, const uint32_
  *   KCL_PAVA_PREPROCESS: reads level[n], acc[level[w]] += 1.0
  *                        (rho_tentative, kCDensestCmpSync.c:239-240)
  * acc is accumulated into, like the reference's (callers zero it).
- * visits (nullable) receives the number of k-cliques.  2 <= k <= KCL_MAX_K.
+ * visits (nullable) receives the number of k-cliques.  2 <= k <= 255.
  * ------------------------------------------------------------------- */
 kcl_status kcl_clique_enum(kcl_graph *g, uint32_t k, kcl_task task, const double *rho,
                            const uint32_t *level, double *acc, uint64_t *visits);
@@ -143,6 +145,20 @@ kcl_status kcl_sim_totals(kcl_sim *s, uint64_t *V);
 /* k-cliques visited by the last pass and its GPU time (HIP events on the
  * pass's stream, enumeration kernels only / whole pass). */
 kcl_status kcl_sim_last_pass(kcl_sim *s, uint64_t *visits, double *kernel_ms, double *pass_ms);
+
+/* The enumeration schedule, fixed per graph and k.  Roots are DAG nodes
+ * with out-degree >= k-1; those with no k-clique are dropped at set-up by
+ * a counting pass (exact); heavy ones are split into several tasks. */
+typedef struct kcl_plan_info {
+  uint32_t roots_total; /* nodes with out-degree >= k-1 */
+  uint32_t roots_kept;  /* of those, roots of at least one k-clique */
+  uint32_t tasks;       /* wavefront tasks per pass (after splitting) */
+  uint32_t reserved;
+  uint64_t cliques;     /* k-cliques in the graph (= visits of every pass) */
+  double plan_ms;       /* set-up time of the plan (counting passes included) */
+} kcl_plan_info;
+
+kcl_status kcl_sim_plan_info(kcl_sim *s, kcl_plan_info *info);
 
 typedef struct kcl_densest_info {
   uint32_t n;       /* nodes of the densest prefix */

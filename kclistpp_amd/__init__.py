@@ -58,6 +58,11 @@ class _Densest(C.Structure):
                 ("ub", C.c_double), ("sum", C.c_uint64)]
 
 
+class _PlanInfo(C.Structure):
+    _fields_ = [("roots_total", C.c_uint32), ("roots_kept", C.c_uint32), ("tasks", C.c_uint32),
+                ("reserved", C.c_uint32), ("cliques", C.c_uint64), ("plan_ms", C.c_double)]
+
+
 class _Record(C.Structure):
     _fields_ = [("T", C.c_uint32), ("n", C.c_uint32), ("m", C.c_uint32),
                 ("density", C.c_double), ("ub", C.c_double), ("sum", C.c_uint64),
@@ -76,7 +81,7 @@ EXPORTS = [
     "kcl_graph_create", "kcl_graph_destroy", "kcl_graph_get_info", "kcl_graph_rank",
     "kcl_graph_csr", "kcl_graph_edges", "kcl_count_edges", "kcl_clique_enum", "kcl_sim_create",
     "kcl_sim_destroy", "kcl_sim_step", "kcl_sim_counts", "kcl_sim_rho", "kcl_sim_set_rho",
-    "kcl_sim_totals", "kcl_sim_last_pass", "kcl_sim_find_densest", "kcl_cmpsync_run",
+    "kcl_sim_totals", "kcl_sim_last_pass", "kcl_sim_plan_info", "kcl_sim_find_densest", "kcl_cmpsync_run",
     "kcl_device_get_info", "kcl_version",
 ]
 
@@ -115,6 +120,7 @@ def lib() -> C.CDLL:
         "kcl_sim_totals": ([vp, vp], C.c_int),
         "kcl_sim_last_pass": ([vp, C.POINTER(u64), C.POINTER(C.c_double),
                                C.POINTER(C.c_double)], C.c_int),
+        "kcl_sim_plan_info": ([vp, C.POINTER(_PlanInfo)], C.c_int),
         "kcl_sim_find_densest": ([vp, C.POINTER(_Densest), vp, vp, vp], C.c_int),
         "kcl_cmpsync_run": ([vp, u32, u32, C.POINTER(_SimOpts), C.c_char_p, i64,
                              C.POINTER(_Record), C.POINTER(u32)], C.c_int),
@@ -324,6 +330,12 @@ class Sim:
         pm = C.c_double(0)
         _check(lib().kcl_sim_last_pass(self._h, C.byref(v), C.byref(km), C.byref(pm)))
         return int(v.value), float(km.value), float(pm.value)
+
+    def plan_info(self) -> dict:
+        p = _PlanInfo()
+        _check(lib().kcl_sim_plan_info(self._h, C.byref(p)))
+        return {"roots_total": p.roots_total, "roots_kept": p.roots_kept, "tasks": p.tasks,
+                "cliques": int(p.cliques), "plan_ms": p.plan_ms}
 
     def find_densest(self, want_vectors: bool = False):
         d = _Densest()

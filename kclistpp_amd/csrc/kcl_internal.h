@@ -61,8 +61,12 @@ struct DevBuf {
   }
 };
 
-// Word-class limits of the enumeration kernels: a root's out-neighbourhood
-// of d nodes is held as d rows of W 64-bit words, W in {1,2,4,8,16}.
+// Word classes of the enumeration kernels: a root's out-neighbourhood of d
+// nodes is held as d rows of W 64-bit words (d <= 64 W), processed by a
+// workgroup of kClassWaves wavefronts.
+constexpr uint32_t kNumClasses = 8;
+constexpr uint32_t kClassWords[kNumClasses] = {1, 2, 3, 4, 6, 8, 12, 16};
+constexpr uint32_t kClassWaves[kNumClasses] = {1, 2, 4, 4, 4, 4, 4, 4};
 constexpr uint32_t kMaxWords = 16;
 constexpr uint32_t kMaxRootDegree = 64 * kMaxWords;
 
@@ -80,15 +84,15 @@ enum PassMode : uint32_t {
 struct PassPlan {
   // Tasks grouped by word class, heaviest first inside each class.
   std::vector<Task> tasks;
-  uint32_t class_begin[6] = {0, 0, 0, 0, 0, 0};  // classes W = 1,2,4,8,16
-  uint32_t class_count[5] = {0, 0, 0, 0, 0};
+  uint32_t class_begin[kNumClasses + 1] = {};
+  uint32_t class_count[kNumClasses] = {};
 };
 
 struct DevicePlan {
   DevBuf<Task> tasks;
   DevBuf<uint32_t> tickets;  // one queue head per class
-  uint32_t class_begin[6] = {0, 0, 0, 0, 0, 0};
-  uint32_t class_count[5] = {0, 0, 0, 0, 0};
+  uint32_t class_begin[kNumClasses + 1] = {};
+  uint32_t class_count[kNumClasses] = {};
 };
 
 struct DeviceProps {
@@ -107,9 +111,10 @@ kcl_status launch_pass(const uint32_t *d_cd, const uint32_t *d_adj, const uint64
                        hipStream_t stream);
 
 // Work-unit estimate per root from a root-credit pass of (k-1)-cliques.
+// keep (nullable): only roots with keep[s] != 0 get tasks.
 kcl_status build_plan(const std::vector<uint32_t> &cd, uint32_t k,
                       const std::vector<uint64_t> *root_cost, uint32_t split_work,
-                      PassPlan *plan);
+                      PassPlan *plan, const std::vector<uint64_t> *keep = nullptr);
 kcl_status upload_plan(const PassPlan &plan, DevicePlan *dev);
 
 }  // namespace kcl

@@ -50,6 +50,8 @@ struct Lds {
   uint32_t node[D];             // member ids, ascending
   uint32_t beg[D];              // cd[member]
   uint32_t pre[D + 1];          // prefix sums (member degrees, later |A'[u]|)
+  unsigned long long root_sum;  // the root's credits, summed over the waves
+  uint32_t task;                // ticket broadcast
 };
 
 __device__ __forceinline__ uint32_t select_bit(unsigned long long x, uint32_t r) {
@@ -132,23 +134,27 @@ __device__ __forceinline__ void wave_scan(uint32_t *a, uint32_t d, uint32_t lane
   }
 }
 
-template <int W, int K>
-__global__ __launch_bounds__(64) void pass_kernel(const uint32_t *__restrict__ cd,
-                                                  const uint32_t *__restrict__ adj,
-                                                  const uint64_t *__restrict__ key,
-                                                  const Task *__restrict__ tasks, uint32_t ntasks,
-                                                  uint32_t *__restrict__ ticket, uint32_t mode,
-                                                  unsigned long long *__restrict__ cnt) {
+template <int W, int K, int NW>
+__global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restrict__ cd,
+                                                       const uint32_t *__restrict__ adj,
+                                                       const uint64_t *__restrict__ key,
+                                                       const Task *__restrict__ tasks,
+                                                       uint32_t ntasks,
+                                                       uint32_t *__restrict__ ticket,
+                                                       uint32_t mode,
+                                                       unsigned long long *__restrict__ cnt) {
   static_assert(K >= 2, "k >= 2");
+  constexpr uint32_t NT = 64 * NW;  // threads of the workgroup sharing one root
   __shared__ Lds<W> sh;
-  const uint32_t lane = threadIdx.x;
+  const uint32_t tid = threadIdx.x;
+  const uint32_t lane = tid & 63;
   const bool to_root = (mode == kCreditRoot);
 
   for (;;) {
-    uint32_t ti = 0;
-    if (lane == 0) ti = atomicAdd(ticket, 1u);
-    ti = __shfl(ti, 0);
-    if (ti >= ntasks) break;
+    if (tid == 0) sh.task = atomicAdd(ticket, 1u);
+    __syncthreads();
+    const uint32_t ti = sh.task;
+    if (ti >= ntasks) break;  // same value in every thread of the workgroup
     const Task tk = tasks[ti];
     const uint32_t s = tk.s;
     const uint32_t part = tk.part_np & 0xffffu, np = tk.part_np >> 16;
@@ -157,7 +163,7 @@ __global__ __launch_bounds__(64) void pass_kernel(const uint32_t *__restrict__ c
     const uint64_t ks = key[s];
 
     // Stage the members, their keys and adjacency extents.
-    for (uint32_t i = lane; i < d; i += 64) {
+    for (uint32_t i = tid; i < d; i += NT) {
       const uint32_t x = adj[b0 + i];
       sh.node[i] = x;
       sh.key[i] = key[x];
@@ -166,9 +172,12 @@ __global__ __launch_bounds__(64) void pass_kernel(const uint32_t *__restrict__ c
       sh.pre[i + 1] = cd[x + 1] - xb;
       sh.cred[i] = 0;
     }
-    if (lane == 0) sh.pre[0] = 0;
+    if (tid == 0) {
+      sh.pre[0] = 0;
+      sh.root_sum = 0;
+    }
     if constexpr (K >= 3) {
-      for (uint32_t i = lane; i < d * W; i += 64) sh.rows[i] = 0;
+      for (uint32_t i = tid; i < d * W; i += NT) sh.rows[i] = 0;
     }
     __syncthreads();
 
@@ -177,17 +186,17 @@ __global__ __launch_bounds__(64) void pass_kernel(const uint32_t *__restrict__ c
       // Cliques {s, u}: items are the members themselves.
       const uint32_t lo = (uint32_t)((uint64_t)part * d / np);
       const uint32_t hi = (uint32_t)((uint64_t)(part + 1) * d / np);
-      for (uint32_t u = lo + lane; u < hi; u += 64) {
+      for (uint32_t u = lo + tid; u < hi; u += NT) {
         if (!to_root && sh.key[u] <= ks) sh.cred[u] += 1;
         else root_acc += 1;
       }
     } else {
       // Oriented rows from the members' adjacency slices.
-      wave_scan(sh.pre + 1, d, lane);
+      if (tid < 64) wave_scan(sh.pre + 1, d, lane);
       __syncthreads();
       const uint32_t total = sh.pre[d];
       uint32_t cur = 0;
-      for (uint32_t e = lane; e < total; e += 64) {
+      for (uint32_t e = tid; e < total; e += NT) {
         uint32_t lo = cur, hi = d;  // member owning element e: pre[i] <= e < pre[i+1]
         while (hi - lo > 1) {
           const uint32_t mid = (lo + hi) >> 1;
@@ -214,7 +223,7 @@ __global__ __launch_bounds__(64) void pass_kernel(const uint32_t *__restrict__ c
       if constexpr (K == 3) {
         const uint32_t lo = (uint32_t)((uint64_t)part * d / np);
         const uint32_t hi = (uint32_t)((uint64_t)(part + 1) * d / np);
-        for (uint32_t u = lo + lane; u < hi; u += 64) {
+        for (uint32_t u = lo + tid; u < hi; u += NT) {
           uint32_t c = 0;
 #pragma unroll
           for (int j = 0; j < W; ++j) c += __popcll(sh.rows[u * W + j]);
@@ -230,21 +239,21 @@ __global__ __launch_bounds__(64) void pass_kernel(const uint32_t *__restrict__ c
         constexpr uint32_t kMinSet = kEdgeItems ? (uint32_t)(K - 3) : 2u;
         uint32_t nitems = d;
         if constexpr (kEdgeItems) {
-          for (uint32_t u = lane; u < d; u += 64) {
+          for (uint32_t u = tid; u < d; u += NT) {
             uint32_t c = 0;
 #pragma unroll
             for (int j = 0; j < W; ++j) c += __popcll(sh.rows[u * W + j]);
             sh.pre[u + 1] = c;
           }
           __syncthreads();
-          wave_scan(sh.pre + 1, d, lane);
+          if (tid < 64) wave_scan(sh.pre + 1, d, lane);
           __syncthreads();
           nitems = sh.pre[d];
         }
         const uint32_t ilo = (uint32_t)((uint64_t)part * nitems / np);
         const uint32_t ihi = (uint32_t)((uint64_t)(part + 1) * nitems / np);
 
-        uint32_t e = ilo + lane;
+        uint32_t e = ilo + tid;
         uint32_t u = 0, wi = 0;
         unsigned long long P[W];
         unsigned long long bits = 0;
@@ -257,7 +266,7 @@ __global__ __launch_bounds__(64) void pass_kernel(const uint32_t *__restrict__ c
           live = false;
           while (e < ihi) {
             const uint32_t it = e;
-            e += 64;
+            e += NT;
             if constexpr (kEdgeItems) {
               uint32_t lo = ucur, hi = d;  // pre[u] <= it < pre[u+1]
               while (hi - lo > 1) {
@@ -327,11 +336,13 @@ __global__ __launch_bounds__(64) void pass_kernel(const uint32_t *__restrict__ c
       }
     }
 
-    // Flush: the root's share (wave-reduced), then each member's.
+    // Flush: the root's share (reduced per wave, then per workgroup), then
+    // each member's.
     root_acc = wave_sum(root_acc);
-    if (lane == 0 && root_acc) atomicAdd(&cnt[s], root_acc);
+    if (lane == 0 && root_acc) atomicAdd(&sh.root_sum, root_acc);
     __syncthreads();
-    for (uint32_t i = lane; i < d; i += 64) {
+    if (tid == 0 && sh.root_sum) atomicAdd(&cnt[s], sh.root_sum);
+    for (uint32_t i = tid; i < d; i += NT) {
       const unsigned long long c = sh.cred[i];
       if (c) atomicAdd(&cnt[sh.node[i]], c);
     }
@@ -344,40 +355,42 @@ using PassFn = void (*)(const uint32_t *, const uint32_t *, const uint64_t *, co
 
 constexpr int kMaxTemplK = 10;  // k handled by compiled kernels: 2..kMaxTemplK
 
-template <int W, int K>
+template <int W, int NW, int K>
 struct KTable {
   static void fill(PassFn *t) {
-    t[K] = &pass_kernel<W, K>;
-    if constexpr (K < kMaxTemplK) KTable<W, K + 1>::fill(t);
+    t[K] = &pass_kernel<W, K, NW>;
+    if constexpr (K < kMaxTemplK) KTable<W, NW, K + 1>::fill(t);
+  }
+};
+
+template <int C>
+struct ClassTable {
+  static void fill(PassFn (*fn)[kMaxTemplK + 1], size_t *lds) {
+    constexpr int W = (int)kClassWords[C];
+    constexpr int NW = (int)kClassWaves[C];
+    KTable<W, NW, 2>::fill(fn[C]);
+    lds[C] = sizeof(Lds<W>);
+    if constexpr (C + 1 < (int)kNumClasses) ClassTable<C + 1>::fill(fn, lds);
   }
 };
 
 struct PassTable {
-  PassFn fn[5][kMaxTemplK + 1] = {};
-  PassTable() {
-    KTable<1, 2>::fill(fn[0]);
-    KTable<2, 2>::fill(fn[1]);
-    KTable<4, 2>::fill(fn[2]);
-    KTable<8, 2>::fill(fn[3]);
-    KTable<16, 2>::fill(fn[4]);
-  }
+  PassFn fn[kNumClasses][kMaxTemplK + 1] = {};
+  size_t lds[kNumClasses] = {};
+  PassTable() { ClassTable<0>::fill(fn, lds); }
 };
 
-PassFn kernel_for(uint32_t wclass, uint32_t k) {
+const PassTable &pass_table() {
   static const PassTable table;  // thread-safe static initialisation
-  if (wclass > 4 || k < 2 || k > (uint32_t)kMaxTemplK) return nullptr;
-  return table.fn[wclass][k];
+  return table;
 }
 
-size_t lds_bytes(uint32_t wclass) {
-  switch (wclass) {
-    case 0: return sizeof(Lds<1>);
-    case 1: return sizeof(Lds<2>);
-    case 2: return sizeof(Lds<4>);
-    case 3: return sizeof(Lds<8>);
-    default: return sizeof(Lds<16>);
-  }
+PassFn kernel_for(uint32_t wclass, uint32_t k) {
+  if (wclass >= kNumClasses || k < 2 || k > (uint32_t)kMaxTemplK) return nullptr;
+  return pass_table().fn[wclass][k];
 }
+
+size_t lds_bytes(uint32_t wclass) { return pass_table().lds[wclass]; }
 
 }  // namespace
 
@@ -388,19 +401,20 @@ kcl_status launch_pass(const uint32_t *d_cd, const uint32_t *d_adj, const uint64
                        hipStream_t stream) {
   DeviceProps props;
   KCL_TRY(device_props(&props));
-  KCL_HIP(hipMemsetAsync(plan.tickets.p, 0, 8 * sizeof(uint32_t), stream));
-  for (int c = 4; c >= 0; --c) {  // heaviest word class first
+  KCL_HIP(hipMemsetAsync(plan.tickets.p, 0, kNumClasses * sizeof(uint32_t), stream));
+  for (int c = (int)kNumClasses - 1; c >= 0; --c) {  // heaviest word class first
     const uint32_t count = plan.class_count[c];
     if (!count) continue;
     PassFn fn = kernel_for((uint32_t)c, k);
     if (!fn) return fail(KCL_E_UNSUPPORTED, "no enumeration kernel for k = %u", k);
     const size_t lds = lds_bytes((uint32_t)c);
+    const uint32_t waves = kClassWaves[c];
     uint32_t per_cu = (uint32_t)(props.lds_per_cu / (lds + 256));
     if (per_cu < 1) per_cu = 1;
-    if (per_cu > 16) per_cu = 16;
+    if (per_cu > 32 / waves) per_cu = 32 / waves;
     uint64_t blocks = (uint64_t)props.cu_count * per_cu;
     if (blocks > count) blocks = count;
-    hipLaunchKernelGGL(fn, dim3((uint32_t)blocks), dim3(64), 0, stream, d_cd, d_adj, d_key,
+    hipLaunchKernelGGL(fn, dim3((uint32_t)blocks), dim3(64 * waves), 0, stream, d_cd, d_adj, d_key,
                        plan.tasks.p + plan.class_begin[c], count, plan.tickets.p + c,
                        (uint32_t)mode, d_cnt);
     KCL_HIP(hipGetLastError());
@@ -410,24 +424,26 @@ kcl_status launch_pass(const uint32_t *d_cd, const uint32_t *d_adj, const uint64
 
 kcl_status build_plan(const std::vector<uint32_t> &cd, uint32_t k,
                       const std::vector<uint64_t> *root_cost, uint32_t split_work,
-                      PassPlan *plan) {
+                      PassPlan *plan, const std::vector<uint64_t> *keep) {
   const uint32_t n = cd.empty() ? 0 : (uint32_t)(cd.size() - 1);
   struct Entry {
     uint64_t cost;
     uint32_t s;
     uint32_t np;
   };
-  std::vector<Entry> cls[5];
+  std::vector<Entry> cls[kNumClasses];
   const uint32_t need = k - 1;  // members needed in N+(s)
   for (uint32_t s = 0; s < n; ++s) {
     const uint32_t d = cd[s + 1] - cd[s];
     if (d < need || d == 0) continue;
+    if (keep && (*keep)[s] == 0) continue;
     if (d > kMaxRootDegree)
       return fail(KCL_E_UNSUPPORTED,
                   "node %u has out-degree %u in the DAG; the enumerator handles up to %u", s, d,
                   kMaxRootDegree);
     const uint32_t words = (d + 63) / 64;
-    const uint32_t c = words <= 1 ? 0 : words <= 2 ? 1 : words <= 4 ? 2 : words <= 8 ? 3 : 4;
+    uint32_t c = 0;
+    while (kClassWords[c] < words) ++c;  // smallest class that holds d members
     uint64_t cost = root_cost ? (*root_cost)[s] : (uint64_t)d * d;
     uint32_t np = 1;
     if (split_work && cost > split_work) {
@@ -441,7 +457,7 @@ kcl_status build_plan(const std::vector<uint32_t> &cd, uint32_t k,
   }
   plan->tasks.clear();
   uint32_t at = 0;
-  for (int c = 0; c < 5; ++c) {
+  for (uint32_t c = 0; c < kNumClasses; ++c) {
     std::stable_sort(cls[c].begin(), cls[c].end(),
                      [](const Entry &a, const Entry &b) { return a.cost > b.cost; });
     plan->class_begin[c] = at;
@@ -454,18 +470,18 @@ kcl_status build_plan(const std::vector<uint32_t> &cd, uint32_t k,
     plan->class_count[c] = cnt;
     at += cnt;
   }
-  plan->class_begin[5] = at;
+  plan->class_begin[kNumClasses] = at;
   return KCL_OK;
 }
 
 kcl_status upload_plan(const PassPlan &plan, DevicePlan *dev) {
   KCL_TRY(dev->tasks.alloc(plan.tasks.size()));
-  KCL_TRY(dev->tickets.alloc(8));
+  KCL_TRY(dev->tickets.alloc(kNumClasses));
   if (!plan.tasks.empty())
     KCL_HIP(hipMemcpy(dev->tasks.p, plan.tasks.data(), plan.tasks.size() * sizeof(Task),
                       hipMemcpyHostToDevice));
-  for (int c = 0; c < 6; ++c) dev->class_begin[c] = plan.class_begin[c];
-  for (int c = 0; c < 5; ++c) dev->class_count[c] = plan.class_count[c];
+  for (uint32_t c = 0; c <= kNumClasses; ++c) dev->class_begin[c] = plan.class_begin[c];
+  for (uint32_t c = 0; c < kNumClasses; ++c) dev->class_count[c] = plan.class_count[c];
   return KCL_OK;
 }
 

@@ -47,6 +47,7 @@ struct kcl_sim {
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
   uint64_t last_visits = 0;
   double last_kernel_ms = 0, last_pass_ms = 0;
+  kcl_plan_info info{};
 };
 
 namespace kcl {
@@ -162,27 +163,35 @@ kcl_status zero_u64(unsigned long long *p, uint64_t n, hipStream_t st) {
   return KCL_OK;
 }
 
-// Work per root for the plan: a root-credit pass of (k-1)-cliques counts,
-// for every root, the steps its k-pass will take.
-kcl_status root_costs(kcl_graph *g, uint32_t k, hipStream_t st, std::vector<uint64_t> *out) {
+// Per-root clique counts: a root-credit pass of k-cliques over `plan`
+// (out[s] = number of k-cliques whose largest id is s).
+kcl_status root_counts(kcl_graph *g, uint32_t k, const PassPlan &plan, hipStream_t st,
+                       std::vector<uint64_t> *out) {
   out->assign(g->n, 0);
-  PassPlan p;
-  KCL_TRY(build_plan(g->cd, k - 1, nullptr, 0, &p));
   DevicePlan dp;
-  KCL_TRY(upload_plan(p, &dp));
+  KCL_TRY(upload_plan(plan, &dp));
   DevBuf<unsigned long long> c;
   DevBuf<unsigned long long> key;
   KCL_TRY(c.alloc(g->n));
   KCL_TRY(key.alloc(g->n));
   KCL_TRY(zero_u64(c.p, g->n, st));
   KCL_TRY(zero_u64(key.p, g->n, st));
-  KCL_TRY(launch_pass(g->d_cd.p, g->d_adj.p, (const uint64_t *)key.p, k - 1, kCreditRoot, dp,
-                      c.p, st));
+  KCL_TRY(launch_pass(g->d_cd.p, g->d_adj.p, (const uint64_t *)key.p, k, kCreditRoot, dp, c.p,
+                      st));
   KCL_HIP(hipMemcpyAsync(out->data(), c.p, (size_t)g->n * sizeof(uint64_t),
                          hipMemcpyDeviceToHost, st));
   KCL_HIP(hipStreamSynchronize(st));
-  // Add the build cost (adjacency entries scanned) so light roots keep a
-  // weight and split parts amortise their rebuild.
+  return KCL_OK;
+}
+
+// Work per root: the steps of its k-pass (one per (k-1)-clique with that
+// root, counted by a root-credit (k-1)-pass) times the row width, plus the
+// adjacency entries its build scans (so light roots keep a weight and split
+// parts amortise their rebuild).
+kcl_status root_costs(kcl_graph *g, uint32_t k, hipStream_t st, std::vector<uint64_t> *out) {
+  PassPlan p;
+  KCL_TRY(build_plan(g->cd, k - 1, nullptr, 0, &p));
+  KCL_TRY(root_counts(g, k - 1, p, st, out));
   for (uint32_t s = 0; s < g->n; ++s) {
     uint64_t b = 0;
     for (uint32_t j = g->cd[s]; j < g->cd[s + 1]; ++j) {
@@ -215,16 +224,40 @@ kcl_status sim_prepare(kcl_sim *s) {
   KCL_TRY(zero_u64(s->last.p, n, s->stream));
   KCL_TRY(zero_u64(s->V.p, n, s->stream));
   if (!s->trivial) {
+    // The plan is fixed per graph and k (rho only changes who wins):
+    //  1. cost per root from a root-credit (k-1)-pass (k >= 4);
+    //  2. a root-credit k-pass gives each root's k-clique count; roots with
+    //     none are dropped (exact: they contribute nothing to any pass);
+    //  3. tasks for the remaining roots, heavy ones split by cost.
+    const auto t0 = std::chrono::steady_clock::now();
     std::vector<uint64_t> cost;
     const std::vector<uint64_t> *cp = nullptr;
     if (s->k >= 4) {
       KCL_TRY(root_costs(g, s->k, s->stream, &cost));
       cp = &cost;
     }
-    uint32_t split = s->opts.split_work ? s->opts.split_work : (1u << 17);
+    const uint32_t split = s->opts.split_work ? s->opts.split_work : (1u << 17);
+    PassPlan all;
+    KCL_TRY(build_plan(g->cd, s->k, cp, split, &all));
+    std::vector<uint64_t> ck;
+    KCL_TRY(root_counts(g, s->k, all, s->stream, &ck));
     PassPlan p;
-    KCL_TRY(build_plan(g->cd, s->k, cp, split, &p));
+    KCL_TRY(build_plan(g->cd, s->k, cp, split, &p, &ck));
     KCL_TRY(upload_plan(p, &s->plan));
+    uint64_t cliques = 0;
+    uint32_t kept = 0;
+    for (uint32_t v = 0; v < n; ++v) {
+      cliques += ck[v];
+      kept += ck[v] ? 1u : 0u;
+    }
+    uint32_t roots_all = 0;
+    for (uint32_t v = 0; v < n; ++v) roots_all += (g->cd[v + 1] - g->cd[v] >= s->k - 1) ? 1u : 0u;
+    s->info.roots_total = roots_all;
+    s->info.roots_kept = kept;
+    s->info.tasks = (uint32_t)p.tasks.size();
+    s->info.cliques = cliques;
+    s->info.plan_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   }
   KCL_HIP(hipStreamSynchronize(s->stream));
   return KCL_OK;
@@ -458,6 +491,12 @@ extern "C" kcl_status kcl_sim_last_pass(kcl_sim *s, uint64_t *visits, double *ke
   if (visits) *visits = s->last_visits;
   if (kernel_ms) *kernel_ms = s->last_kernel_ms;
   if (pass_ms) *pass_ms = s->last_pass_ms;
+  return KCL_OK;
+}
+
+extern "C" kcl_status kcl_sim_plan_info(kcl_sim *s, kcl_plan_info *info) {
+  if (!s || !info) return fail(KCL_E_INVALID_ARG, "NULL argument");
+  *info = s->info;
   return KCL_OK;
 }
 
