@@ -196,6 +196,50 @@ kcl_status kcl_cmpsync_run(kcl_graph *g, uint32_t k, uint32_t num_iter, const kc
                            uint32_t *nrec);
 
 /* ---------------------------------------------------------------------
+ * k-clique core trimming: kClistCoreTrim (kClistCoreTrim.c:707-775).
+ * (k-1)-core by the reference's degree heap (nodes popped while the running
+ * core is below k-1 are dropped; kept nodes get new ids n2-1, n2-2, ... in
+ * pop order), k-clique degree of every kept node (GPU), then the k-clique
+ * core peel by the reference's min-heap; the trimmed graph keeps the last
+ * ntrim peeled nodes.  3 <= k <= 255 (k <= KCL_MAX_K when cliques exist).
+ * ------------------------------------------------------------------- */
+typedef struct kcl_trim kcl_trim;
+
+typedef struct kcl_trim_info {
+  uint32_t n_in;    /* input nodes (max id + 1) */
+  uint32_t n2;      /* nodes of the (k-1)-core */
+  uint32_t e2;      /* edges among them */
+  uint32_t core;    /* degeneracy of the input */
+  uint64_t ncktot;  /* k-cliques of the (k-1)-core */
+  uint64_t ckcore;  /* k-clique core number */
+  double ckrho;     /* best k-clique density along the peel */
+  double rho;       /* edge density of that subgraph (reference's formula) */
+  uint32_t size;    /* its size */
+  uint32_t ntrim;   /* nodes kept by the trim */
+  uint64_t ncktrim; /* k-cliques of the trimmed graph */
+  uint32_t mtrim;   /* edges of the trimmed graph */
+  uint32_t reserved;
+  double seconds_degrees; /* core + relabel + GPU k-clique degrees */
+  double seconds_peel;    /* host peel */
+} kcl_trim_info;
+
+kcl_status kcl_trim_create(const uint32_t *pairs, uint64_t m, uint32_t n_in, uint32_t k,
+                           kcl_trim **out);
+void kcl_trim_destroy(kcl_trim *t);
+kcl_status kcl_trim_get_info(const kcl_trim *t, kcl_trim_info *info);
+/* Per new id (n2 entries each, every pointer nullable): input id, k-clique
+ * degree (ckdeg), peel order and running core (ckcore), kept flag. */
+kcl_status kcl_trim_nodes(const kcl_trim *t, uint32_t *map, uint64_t *nck, uint32_t *peel_key,
+                          uint64_t *peel_core, uint8_t *kept);
+/* Trimmed edges (2*mtrim input ids, in input order, larger new id first). */
+kcl_status kcl_trim_edges(const kcl_trim *t, uint32_t *pairs);
+/* The reference's five output files (each path nullable): ckdeg, ckcore,
+ * ckdens, trimmed_<tag>_<k>.txt and stat_trim_<tag>_<k>.txt. */
+kcl_status kcl_trim_write(const kcl_trim *t, const char *ckdeg, const char *ckcore,
+                          const char *ckdens, const char *trimmed, const char *stat,
+                          int64_t t0_unix);
+
+/* ---------------------------------------------------------------------
  * Library information.
  * ------------------------------------------------------------------- */
 typedef struct kcl_device_info {

@@ -63,6 +63,15 @@ class _PlanInfo(C.Structure):
                 ("reserved", C.c_uint32), ("cliques", C.c_uint64), ("plan_ms", C.c_double)]
 
 
+class _TrimInfo(C.Structure):
+    _fields_ = [("n_in", C.c_uint32), ("n2", C.c_uint32), ("e2", C.c_uint32),
+                ("core", C.c_uint32), ("ncktot", C.c_uint64), ("ckcore", C.c_uint64),
+                ("ckrho", C.c_double), ("rho", C.c_double), ("size", C.c_uint32),
+                ("ntrim", C.c_uint32), ("ncktrim", C.c_uint64), ("mtrim", C.c_uint32),
+                ("reserved", C.c_uint32), ("seconds_degrees", C.c_double),
+                ("seconds_peel", C.c_double)]
+
+
 class _Record(C.Structure):
     _fields_ = [("T", C.c_uint32), ("n", C.c_uint32), ("m", C.c_uint32),
                 ("density", C.c_double), ("ub", C.c_double), ("sum", C.c_uint64),
@@ -82,7 +91,8 @@ EXPORTS = [
     "kcl_graph_csr", "kcl_graph_edges", "kcl_count_edges", "kcl_clique_enum", "kcl_sim_create",
     "kcl_sim_destroy", "kcl_sim_step", "kcl_sim_counts", "kcl_sim_rho", "kcl_sim_set_rho",
     "kcl_sim_totals", "kcl_sim_last_pass", "kcl_sim_plan_info", "kcl_sim_find_densest", "kcl_cmpsync_run",
-    "kcl_device_get_info", "kcl_version",
+    "kcl_device_get_info", "kcl_version", "kcl_trim_create", "kcl_trim_destroy",
+    "kcl_trim_get_info", "kcl_trim_nodes", "kcl_trim_edges", "kcl_trim_write",
 ]
 
 
@@ -125,6 +135,13 @@ def lib() -> C.CDLL:
         "kcl_cmpsync_run": ([vp, u32, u32, C.POINTER(_SimOpts), C.c_char_p, i64,
                              C.POINTER(_Record), C.POINTER(u32)], C.c_int),
         "kcl_device_get_info": ([C.POINTER(_DeviceInfo)], C.c_int),
+        "kcl_trim_create": ([vp, u64, u32, u32, C.POINTER(vp)], C.c_int),
+        "kcl_trim_destroy": ([vp], None),
+        "kcl_trim_get_info": ([vp, C.POINTER(_TrimInfo)], C.c_int),
+        "kcl_trim_nodes": ([vp, vp, vp, vp, vp, vp], C.c_int),
+        "kcl_trim_edges": ([vp, vp], C.c_int),
+        "kcl_trim_write": ([vp, C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p,
+                            i64], C.c_int),
     }
     for name, (args, res) in sigs.items():
         f = getattr(L, name)
@@ -362,3 +379,59 @@ def cmpsync(graph: Graph, k: int, num_iter: int, stat_path: str | None = None,
                                  C.byref(nrec)))
     return [{"T": r.T, "n": r.n, "m": r.m, "density": r.density, "ub": r.ub, "sum": r.sum,
              "seconds": r.seconds} for r in recs[:nrec.value]]
+
+
+class Trim:
+    """k-clique core trimming (kClistCoreTrim): k-clique degrees on the GPU,
+    the reference's heaps on the host."""
+
+    def __init__(self, pairs: np.ndarray, k: int, n_in: int = 0):
+        pairs = np.ascontiguousarray(pairs, dtype=np.uint32).reshape(-1, 2)
+        h = C.c_void_p()
+        _check(lib().kcl_trim_create(_ptr(pairs) if pairs.size else None, pairs.shape[0], n_in,
+                                     k, C.byref(h)))
+        self._h = h
+        self.k = k
+        info = _TrimInfo()
+        _check(lib().kcl_trim_get_info(h, C.byref(info)))
+        self.info = {f: getattr(info, f) for f, _ in _TrimInfo._fields_ if f != "reserved"}
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().kcl_trim_destroy(self._h)
+            self._h = None
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def nodes(self):
+        """(map, nck, peel_key, peel_core, kept), n2 entries each."""
+        n2 = self.info["n2"]
+        m = max(n2, 1)
+        mp = np.zeros(m, np.uint32)
+        nck = np.zeros(m, np.uint64)
+        pk = np.zeros(m, np.uint32)
+        pc = np.zeros(m, np.uint64)
+        kept = np.zeros(m, np.uint8)
+        _check(lib().kcl_trim_nodes(self._h, _ptr(mp), _ptr(nck), _ptr(pk), _ptr(pc), _ptr(kept)))
+        return mp[:n2], nck[:n2], pk[:n2], pc[:n2], kept[:n2]
+
+    def edges(self) -> np.ndarray:
+        mt = self.info["mtrim"]
+        out = np.zeros(max(2 * mt, 2), np.uint32)
+        _check(lib().kcl_trim_edges(self._h, _ptr(out)))
+        return out[:2 * mt].reshape(-1, 2)
+
+    def write(self, ckdeg=None, ckcore=None, ckdens=None, trimmed=None, stat=None, t0_unix=0):
+        enc = lambda p: p.encode() if p else None  # noqa: E731
+        _check(lib().kcl_trim_write(self._h, enc(ckdeg), enc(ckcore), enc(ckdens), enc(trimmed),
+                                    enc(stat), t0_unix))

@@ -35,9 +35,10 @@ HIP_FLAGS = [
     "-Wall", "-Wno-unused-function", "-I", INCLUDE, "-I", CSRC,
 ]
 
-SOURCES = ["host_io.cpp", "host_order.cpp", "graph.hip", "pass.hip", "sim.hip"]
+SOURCES = ["host_io.cpp", "host_order.cpp", "graph.hip", "pass.hip", "sim.hip", "trim.cpp"]
 CLIS = {
     "kCDensestCmpSync": "cli_cmpsync.c",
+    "kClistCoreTrim": "cli_trim.c",
 }
 
 
@@ -48,18 +49,27 @@ def _run(cmd):
     return r
 
 
-def _compile(src: str) -> str:
+# One object per word class of the enumeration kernels (compiled in parallel).
+PASS_CLASSES = 8
+
+
+def _compile(job) -> str:
+    src, defines, stem = job
     path = os.path.join(CSRC, src)
-    obj = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
+    obj = os.path.join(OBJ, stem + ".o")
     lang = ["-x", "hip"] if src.endswith(".hip") else []
-    _run([HIPCC, *HIP_FLAGS, *lang, "-c", path, "-o", obj])
+    _run([HIPCC, *HIP_FLAGS, *defines, *lang, "-c", path, "-o", obj])
     return obj
 
 
 def build_lib(jobs: int = 8) -> str:
     os.makedirs(OBJ, exist_ok=True)
+    work = [(src, [], os.path.splitext(src)[0]) for src in SOURCES]
+    work += [("pass_inst.hip", [f"-DKCL_PASS_CLASS={c}"], f"pass_inst_{c}")
+             for c in range(PASS_CLASSES)]
+    work.sort(key=lambda w: 0 if w[0] == "pass_inst.hip" else 1)  # longest first
     with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
-        objs = list(ex.map(_compile, SOURCES))
+        objs = list(ex.map(_compile, work))
     tmp = LIB + ".tmp"
     _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp, *objs])
     os.replace(tmp, LIB)

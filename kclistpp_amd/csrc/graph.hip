@@ -162,6 +162,11 @@ using kcl::fail;
 
 extern "C" kcl_status kcl_graph_create(const uint32_t *pairs, uint64_t m, uint32_t n_in,
                                        kcl_graph **out) {
+  return kcl::graph_create(pairs, m, n_in, nullptr, out);
+}
+
+kcl_status kcl::graph_create(const uint32_t *pairs, uint64_t m, uint32_t n_in,
+                             const uint32_t *given_rank, kcl_graph **out) {
   if (!out) return fail(KCL_E_INVALID_ARG, "out is NULL");
   *out = nullptr;
   if (m && !pairs) return fail(KCL_E_INVALID_ARG, "pairs is NULL");
@@ -188,7 +193,10 @@ extern "C" kcl_status kcl_graph_create(const uint32_t *pairs, uint64_t m, uint32
     delete g;
     return fail(KCL_E_OOM_HOST, "out of host memory for %u ranks", n_in);
   }
-  kcl::degeneracy_rank(n_in, m, pairs, g->rank.data());
+  if (given_rank)
+    memcpy(g->rank.data(), given_rank, (size_t)n_in * sizeof(uint32_t));
+  else
+    kcl::degeneracy_rank(n_in, m, pairs, g->rank.data());
   hipError_t e = hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
     delete g;

@@ -79,6 +79,7 @@ struct Task {
 enum PassMode : uint32_t {
   kCreditWinner = 0,  // each clique credits its minimum-key member
   kCreditRoot = 1,    // each clique credits its root (per-root clique counts)
+  kCreditAll = 2,     // each clique credits every member (k-clique degrees)
 };
 
 struct PassPlan {
@@ -117,9 +118,23 @@ kcl_status build_plan(const std::vector<uint32_t> &cd, uint32_t k,
                       PassPlan *plan, const std::vector<uint64_t> *keep = nullptr);
 kcl_status upload_plan(const PassPlan &plan, DevicePlan *dev);
 
+// Plan for (graph, k): cost pass, root filter, split tasks.  root_cliques
+// (nullable) receives each root's k-clique count, info (nullable) a summary.
+kcl_status make_plan(kcl_graph *g, uint32_t k, uint32_t split, hipStream_t st, PassPlan *out,
+                     std::vector<uint64_t> *root_cliques, kcl_plan_info *info);
+
+// Largest k with compiled enumeration kernels.
+uint32_t max_pass_k();
+
+// Graph creation with the degeneracy heap (rank = nullptr) or given ranks
+// (a permutation of 0..n_in-1).
+kcl_status graph_create(const uint32_t *pairs, uint64_t m, uint32_t n_in, const uint32_t *rank,
+                        kcl_graph **out);
+
 }  // namespace kcl
 
 // Opaque ABI types.
+struct kcl_graph;
 struct kcl_graph {
   uint32_t n_in = 0;
   uint32_t n = 0;

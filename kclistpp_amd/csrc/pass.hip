@@ -1,383 +1,24 @@
-// pass.hip - the k-clique enumeration pass on gfx950.
-//
-// Replaces CDF_CliqueEnum + MakeSubgraph + CDF_CliqueEnumThread +
-// CDF_FrankWolfeUpdateRates (kCDensestCmpSync.c:71-117, 138-325).
-//
-// Formulation.  Every k-clique Q has a unique largest id s (its root);
-// Q \ {s} is a (k-1)-clique inside L = N+(s), the root's out-neighbours.
-// The reference credits Q to its member of smallest (key, id), where key
-// is rho (Frank-Wolfe) or the reversed level (PAVA).  Orient the graph
-// induced on L by that order (u -> v when (key_u, id_u) < (key_v, id_v));
-// then the winner among Q \ {s} is its source u, and
-//     winner(Q) = u if (key_u, id_u) < (key_s, s) else s.
-// Since every member of L has a smaller id than s, that test is
-// key_u <= key_s.  So the pass needs, for every u in L, the number of
-// (k-2)-cliques inside A'[u] (u's out-set in the oriented graph), and
-// credits it to u or to s.  The same count with all credit sent to s
-// gives per-root clique counts (the cost pre-pass and trimming).
-//
-// Kernel shape.  One wavefront per root task, persistent, taking tasks
-// from a global ticket (tasks are sorted heaviest first).  L (<= 64*W
-// nodes) is staged in LDS with the members' keys; the oriented rows A'
-// are W 64-bit words each, built from the members' adjacency slices
-// (coalesced reads, binary search in L, ds_or into the row of the
-// earlier endpoint).  Counting is a per-lane state machine over items:
-//   k = 2: item u, one clique {s, u}
-//   k = 3: item u, popcount(A'[u])
-//   k = 4: item u,       P = A'[u],          steps w in P: popcount(P & A'[w])
-//   k >= 5: item (u, v),  P = A'[u] & A'[v], steps w in P: cliques of size
-//           k-4 inside P & A'[w] (nested loops, compile-time depth)
-// A lane that finishes an item takes its next one at once, so lanes stay
-// busy until the root's items run out.  Credits gather in LDS (u64 per
-// member) and are flushed with one global atomicAdd per non-zero member,
-// plus one wave-reduced atomicAdd for the root.
-//
-// Heavy roots are split into parts: part p of np takes items
-// [p*I/np, (p+1)*I/np) and rebuilds A' (the rebuild is cheap next to the
-// counting it splits).
-#include "kcl_internal.h"
+// pass.hip - host side of the enumeration pass: kernel table, launches,
+// task plans.  The kernels are in pass_kernel.h (one instantiation unit per
+// word class, pass_inst.hip).
+#include <algorithm>
+
+#include "pass_kernel.h"
 
 namespace kcl {
 
 namespace {
 
-template <int W>
-struct Lds {
-  static constexpr int D = 64 * W;
-  unsigned long long rows[D * W];  // oriented rows A'
-  uint64_t key[D];                 // member keys
-  unsigned long long cred[D];   // credits gathered per member
-  uint32_t node[D];             // member ids, ascending
-  uint32_t beg[D];              // cd[member]
-  uint32_t pre[D + 1];          // prefix sums (member degrees, later |A'[u]|)
-  unsigned long long root_sum;  // the root's credits, summed over the waves
-  uint32_t task;                // ticket broadcast
-};
-
-__device__ __forceinline__ uint32_t select_bit(unsigned long long x, uint32_t r) {
-  // Position of the r-th (0-based) set bit of x; r < popcount(x).
-  uint32_t pos = 0, c;
-  c = __popcll(x & 0xffffffffull);
-  if (r >= c) { r -= c; x >>= 32; pos += 32; }
-  c = __popc((uint32_t)x & 0xffffu);
-  if (r >= c) { r -= c; x >>= 16; pos += 16; }
-  c = __popc((uint32_t)x & 0xffu);
-  if (r >= c) { r -= c; x >>= 8; pos += 8; }
-  c = __popc((uint32_t)x & 0xfu);
-  if (r >= c) { r -= c; x >>= 4; pos += 4; }
-  c = __popc((uint32_t)x & 0x3u);
-  if (r >= c) { r -= c; x >>= 2; pos += 2; }
-  c = (uint32_t)x & 1u;
-  if (r >= c) pos += 1;
-  return pos;
-}
-
-template <int W>
-__device__ __forceinline__ uint32_t popc_set(const unsigned long long (&p)[W]) {
-  uint32_t c = 0;
-#pragma unroll
-  for (int i = 0; i < W; ++i) c += __popcll(p[i]);
-  return c;
-}
-
-// Number of J-cliques inside P (P is a set of members; cliques follow the
-// orientation, so each is counted once, from its source).
-template <int W, int J>
-__device__ __forceinline__ uint64_t cliques_in(const unsigned long long (&P)[W],
-                                               const unsigned long long *rows) {
-  if constexpr (J == 1) {
-    return popc_set<W>(P);
-  } else {
-    uint64_t total = 0;
-#pragma unroll
-    for (int wi = 0; wi < W; ++wi) {
-      uint64_t bits = P[wi];
-      while (bits) {
-        const uint32_t b = (uint32_t)__builtin_ctzll(bits);
-        bits &= bits - 1;
-        const unsigned long long *r = rows + (uint32_t)(wi * 64 + b) * W;
-        unsigned long long Q[W];
-#pragma unroll
-        for (int j = 0; j < W; ++j) Q[j] = P[j] & r[j];
-        if constexpr (J >= 3) {
-          if (popc_set<W>(Q) < (uint32_t)(J - 1)) continue;
-        }
-        total += cliques_in<W, J - 1>(Q, rows);
-      }
-    }
-    return total;
-  }
-}
-
-__device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-  return v;
-}
-
-// In-place inclusive scan of a[0..d) by one wavefront.
-__device__ __forceinline__ void wave_scan(uint32_t *a, uint32_t d, uint32_t lane) {
-  const uint32_t chunk = (d + 63) >> 6;
-  const uint32_t lo = min(d, lane * chunk), hi = min(d, lo + chunk);
-  uint32_t sum = 0;
-  for (uint32_t i = lo; i < hi; ++i) sum += a[i];
-  uint32_t inc = sum;
-#pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    uint32_t v = __shfl_up(inc, off);
-    if (lane >= (uint32_t)off) inc += v;
-  }
-  uint32_t run = inc - sum;
-  for (uint32_t i = lo; i < hi; ++i) {
-    run += a[i];
-    a[i] = run;
-  }
-}
-
-template <int W, int K, int NW>
-__global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restrict__ cd,
-                                                       const uint32_t *__restrict__ adj,
-                                                       const uint64_t *__restrict__ key,
-                                                       const Task *__restrict__ tasks,
-                                                       uint32_t ntasks,
-                                                       uint32_t *__restrict__ ticket,
-                                                       uint32_t mode,
-                                                       unsigned long long *__restrict__ cnt) {
-  static_assert(K >= 2, "k >= 2");
-  constexpr uint32_t NT = 64 * NW;  // threads of the workgroup sharing one root
-  __shared__ Lds<W> sh;
-  const uint32_t tid = threadIdx.x;
-  const uint32_t lane = tid & 63;
-  const bool to_root = (mode == kCreditRoot);
-
-  for (;;) {
-    if (tid == 0) sh.task = atomicAdd(ticket, 1u);
-    __syncthreads();
-    const uint32_t ti = sh.task;
-    if (ti >= ntasks) break;  // same value in every thread of the workgroup
-    const Task tk = tasks[ti];
-    const uint32_t s = tk.s;
-    const uint32_t part = tk.part_np & 0xffffu, np = tk.part_np >> 16;
-    const uint32_t b0 = cd[s];
-    const uint32_t d = cd[s + 1] - b0;
-    const uint64_t ks = key[s];
-
-    // Stage the members, their keys and adjacency extents.
-    for (uint32_t i = tid; i < d; i += NT) {
-      const uint32_t x = adj[b0 + i];
-      sh.node[i] = x;
-      sh.key[i] = key[x];
-      const uint32_t xb = cd[x];
-      sh.beg[i] = xb;
-      sh.pre[i + 1] = cd[x + 1] - xb;
-      sh.cred[i] = 0;
-    }
-    if (tid == 0) {
-      sh.pre[0] = 0;
-      sh.root_sum = 0;
-    }
-    if constexpr (K >= 3) {
-      for (uint32_t i = tid; i < d * W; i += NT) sh.rows[i] = 0;
-    }
-    __syncthreads();
-
-    unsigned long long root_acc = 0;
-    if constexpr (K == 2) {
-      // Cliques {s, u}: items are the members themselves.
-      const uint32_t lo = (uint32_t)((uint64_t)part * d / np);
-      const uint32_t hi = (uint32_t)((uint64_t)(part + 1) * d / np);
-      for (uint32_t u = lo + tid; u < hi; u += NT) {
-        if (!to_root && sh.key[u] <= ks) sh.cred[u] += 1;
-        else root_acc += 1;
-      }
-    } else {
-      // Oriented rows from the members' adjacency slices.
-      if (tid < 64) wave_scan(sh.pre + 1, d, lane);
-      __syncthreads();
-      const uint32_t total = sh.pre[d];
-      uint32_t cur = 0;
-      for (uint32_t e = tid; e < total; e += NT) {
-        uint32_t lo = cur, hi = d;  // member owning element e: pre[i] <= e < pre[i+1]
-        while (hi - lo > 1) {
-          const uint32_t mid = (lo + hi) >> 1;
-          if (sh.pre[mid] <= e) lo = mid;
-          else hi = mid;
-        }
-        cur = lo;
-        const uint32_t i = lo;
-        const uint32_t y = adj[sh.beg[i] + (e - sh.pre[i])];
-        uint32_t a = 0, b = i;  // y among members [0, i) (all smaller ids)
-        while (a < b) {
-          const uint32_t mid = (a + b) >> 1;
-          if (sh.node[mid] < y) a = mid + 1;
-          else b = mid;
-        }
-        if (a < i && sh.node[a] == y) {
-          const bool a_first = sh.key[a] <= sh.key[i];  // ties: smaller id (a) first
-          const uint32_t from = a_first ? a : i, to = a_first ? i : a;
-          atomicOr(&sh.rows[from * W + (to >> 6)], 1ull << (to & 63));
-        }
-      }
-      __syncthreads();
-
-      if constexpr (K == 3) {
-        const uint32_t lo = (uint32_t)((uint64_t)part * d / np);
-        const uint32_t hi = (uint32_t)((uint64_t)(part + 1) * d / np);
-        for (uint32_t u = lo + tid; u < hi; u += NT) {
-          uint32_t c = 0;
-#pragma unroll
-          for (int j = 0; j < W; ++j) c += __popcll(sh.rows[u * W + j]);
-          if (c) {
-            if (!to_root && sh.key[u] <= ks) sh.cred[u] += c;
-            else root_acc += c;
-          }
-        }
-      } else {
-        // Items: u (k = 4) or DAG edges (u, v) (k >= 5).
-        constexpr bool kEdgeItems = (K >= 5);
-        constexpr int kStepDepth = kEdgeItems ? K - 4 : 1;  // cliques counted per step
-        constexpr uint32_t kMinSet = kEdgeItems ? (uint32_t)(K - 3) : 2u;
-        uint32_t nitems = d;
-        if constexpr (kEdgeItems) {
-          for (uint32_t u = tid; u < d; u += NT) {
-            uint32_t c = 0;
-#pragma unroll
-            for (int j = 0; j < W; ++j) c += __popcll(sh.rows[u * W + j]);
-            sh.pre[u + 1] = c;
-          }
-          __syncthreads();
-          if (tid < 64) wave_scan(sh.pre + 1, d, lane);
-          __syncthreads();
-          nitems = sh.pre[d];
-        }
-        const uint32_t ilo = (uint32_t)((uint64_t)part * nitems / np);
-        const uint32_t ihi = (uint32_t)((uint64_t)(part + 1) * nitems / np);
-
-        uint32_t e = ilo + tid;
-        uint32_t u = 0, wi = 0;
-        unsigned long long P[W];
-        unsigned long long bits = 0;
-        unsigned long long acc = 0;
-        bool live = false;
-        uint32_t ucur = 0;
-
-        // Next item with |P| >= kMinSet, or live = false.
-        auto fetch = [&]() {
-          live = false;
-          while (e < ihi) {
-            const uint32_t it = e;
-            e += NT;
-            if constexpr (kEdgeItems) {
-              uint32_t lo = ucur, hi = d;  // pre[u] <= it < pre[u+1]
-              while (hi - lo > 1) {
-                const uint32_t mid = (lo + hi) >> 1;
-                if (sh.pre[mid] <= it) lo = mid;
-                else hi = mid;
-              }
-              ucur = lo;
-              u = lo;
-              uint32_t r = it - sh.pre[u];
-              uint32_t v = 0;
-              bool found = false;
-#pragma unroll
-              for (int j = 0; j < W; ++j) {
-                const unsigned long long word = sh.rows[u * W + j];
-                const uint32_t c = __popcll(word);
-                if (!found) {
-                  if (r < c) {
-                    v = (uint32_t)j * 64 + select_bit(word, r);
-                    found = true;
-                  } else {
-                    r -= c;
-                  }
-                }
-              }
-#pragma unroll
-              for (int j = 0; j < W; ++j) P[j] = sh.rows[u * W + j] & sh.rows[v * W + j];
-            } else {
-              u = it;
-#pragma unroll
-              for (int j = 0; j < W; ++j) P[j] = sh.rows[u * W + j];
-            }
-            if (popc_set<W>(P) >= kMinSet) {
-              wi = 0;
-              bits = P[0];
-              live = true;
-              return;
-            }
-          }
-        };
-
-        fetch();
-        while (live) {
-          if (bits) {
-            const uint32_t w = wi * 64 + (uint32_t)__builtin_ctzll(bits);
-            bits &= bits - 1;
-            const unsigned long long *r = sh.rows + w * W;
-            unsigned long long Q[W];
-#pragma unroll
-            for (int j = 0; j < W; ++j) Q[j] = P[j] & r[j];
-            acc += cliques_in<W, kStepDepth>(Q, sh.rows);
-          } else if (wi + 1 < (uint32_t)W) {
-            ++wi;
-            unsigned long long nb = 0;
-#pragma unroll
-            for (int j = 0; j < W; ++j) nb = (j == (int)wi) ? P[j] : nb;
-            bits = nb;
-          } else {
-            if (acc) {
-              if (!to_root && sh.key[u] <= ks) atomicAdd(&sh.cred[u], acc);
-              else root_acc += acc;
-              acc = 0;
-            }
-            fetch();
-          }
-        }
-      }
-    }
-
-    // Flush: the root's share (reduced per wave, then per workgroup), then
-    // each member's.
-    root_acc = wave_sum(root_acc);
-    if (lane == 0 && root_acc) atomicAdd(&sh.root_sum, root_acc);
-    __syncthreads();
-    if (tid == 0 && sh.root_sum) atomicAdd(&cnt[s], sh.root_sum);
-    for (uint32_t i = tid; i < d; i += NT) {
-      const unsigned long long c = sh.cred[i];
-      if (c) atomicAdd(&cnt[sh.node[i]], c);
-    }
-    __syncthreads();
-  }
-}
-
-using PassFn = void (*)(const uint32_t *, const uint32_t *, const uint64_t *, const Task *,
-                        uint32_t, uint32_t *, uint32_t, unsigned long long *);
-
-constexpr int kMaxTemplK = 10;  // k handled by compiled kernels: 2..kMaxTemplK
-
-template <int W, int NW, int K>
-struct KTable {
-  static void fill(PassFn *t) {
-    t[K] = &pass_kernel<W, K, NW>;
-    if constexpr (K < kMaxTemplK) KTable<W, NW, K + 1>::fill(t);
-  }
-};
-
-template <int C>
-struct ClassTable {
-  static void fill(PassFn (*fn)[kMaxTemplK + 1], size_t *lds) {
-    constexpr int W = (int)kClassWords[C];
-    constexpr int NW = (int)kClassWaves[C];
-    KTable<W, NW, 2>::fill(fn[C]);
-    lds[C] = sizeof(Lds<W>);
-    if constexpr (C + 1 < (int)kNumClasses) ClassTable<C + 1>::fill(fn, lds);
-  }
-};
-
 struct PassTable {
   PassFn fn[kNumClasses][kMaxTemplK + 1] = {};
+  PassFn fn_all[kNumClasses][kMaxTemplK + 1] = {};
   size_t lds[kNumClasses] = {};
-  PassTable() { ClassTable<0>::fill(fn, lds); }
+  PassTable() {
+    void (*fill[kNumClasses])(PassFn *, PassFn *, size_t *) = {
+        fill_pass_class_0, fill_pass_class_1, fill_pass_class_2, fill_pass_class_3,
+        fill_pass_class_4, fill_pass_class_5, fill_pass_class_6, fill_pass_class_7};
+    for (uint32_t c = 0; c < kNumClasses; ++c) fill[c](fn[c], fn_all[c], &lds[c]);
+  }
 };
 
 const PassTable &pass_table() {
@@ -385,9 +26,9 @@ const PassTable &pass_table() {
   return table;
 }
 
-PassFn kernel_for(uint32_t wclass, uint32_t k) {
+PassFn kernel_for(uint32_t wclass, uint32_t k, bool all) {
   if (wclass >= kNumClasses || k < 2 || k > (uint32_t)kMaxTemplK) return nullptr;
-  return pass_table().fn[wclass][k];
+  return all ? pass_table().fn_all[wclass][k] : pass_table().fn[wclass][k];
 }
 
 size_t lds_bytes(uint32_t wclass) { return pass_table().lds[wclass]; }
@@ -405,7 +46,7 @@ kcl_status launch_pass(const uint32_t *d_cd, const uint32_t *d_adj, const uint64
   for (int c = (int)kNumClasses - 1; c >= 0; --c) {  // heaviest word class first
     const uint32_t count = plan.class_count[c];
     if (!count) continue;
-    PassFn fn = kernel_for((uint32_t)c, k);
+    PassFn fn = kernel_for((uint32_t)c, k, mode == kCreditAll);
     if (!fn) return fail(KCL_E_UNSUPPORTED, "no enumeration kernel for k = %u", k);
     const size_t lds = lds_bytes((uint32_t)c);
     const uint32_t waves = kClassWaves[c];

@@ -1,0 +1,35 @@
+// pass_inst.hip - instantiates the enumeration kernels of one word class.
+// Built once per class with -DKCL_PASS_CLASS=<c> (c = 0..7).
+#include "pass_kernel.h"
+
+#ifndef KCL_PASS_CLASS
+#error "build with -DKCL_PASS_CLASS=<word class index>"
+#endif
+
+namespace kcl {
+
+namespace {
+
+template <int W, int NW, int K>
+struct KTable {
+  static void fill(PassFn *t, PassFn *t_all) {
+    t[K] = &pass_kernel<W, K, NW, false>;
+    t_all[K] = &pass_kernel<W, K, NW, true>;
+    if constexpr (K < kMaxTemplK) KTable<W, NW, K + 1>::fill(t, t_all);
+  }
+};
+
+}  // namespace
+
+#define KCL_CAT2(a, b) a##b
+#define KCL_CAT(a, b) KCL_CAT2(a, b)
+
+void KCL_CAT(fill_pass_class_, KCL_PASS_CLASS)(PassFn *fn, PassFn *fn_all, size_t *lds) {
+  constexpr int C = KCL_PASS_CLASS;
+  constexpr int W = (int)kClassWords[C];
+  constexpr int NW = (int)kClassWaves[C];
+  KTable<W, NW, 2>::fill(fn, fn_all);
+  *lds = sizeof(Lds<W>);
+}
+
+}  // namespace kcl

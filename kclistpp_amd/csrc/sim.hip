@@ -205,6 +205,45 @@ kcl_status root_costs(kcl_graph *g, uint32_t k, hipStream_t st, std::vector<uint
 
 }  // namespace
 
+// The plan is fixed per graph and k (keys only change who wins):
+//  1. cost per root from a root-credit (k-1)-pass (k >= 4);
+//  2. a root-credit k-pass gives each root's k-clique count; roots with
+//     none are dropped (exact: they contribute nothing to any pass);
+//  3. tasks for the remaining roots, heavy ones split by cost.
+kcl_status make_plan(kcl_graph *g, uint32_t k, uint32_t split, hipStream_t st, PassPlan *out,
+                     std::vector<uint64_t> *root_cliques, kcl_plan_info *info) {
+  const auto t0 = std::chrono::steady_clock::now();
+  const uint32_t n = g->n;
+  std::vector<uint64_t> cost;
+  const std::vector<uint64_t> *cp = nullptr;
+  if (k >= 4) {
+    KCL_TRY(root_costs(g, k, st, &cost));
+    cp = &cost;
+  }
+  PassPlan all;
+  KCL_TRY(build_plan(g->cd, k, cp, split, &all));
+  std::vector<uint64_t> ck;
+  KCL_TRY(root_counts(g, k, all, st, &ck));
+  KCL_TRY(build_plan(g->cd, k, cp, split, out, &ck));
+  if (info) {
+    uint64_t cliques = 0;
+    uint32_t kept = 0, roots_all = 0;
+    for (uint32_t v = 0; v < n; ++v) {
+      cliques += ck[v];
+      kept += ck[v] ? 1u : 0u;
+      roots_all += (g->cd[v + 1] - g->cd[v] >= k - 1) ? 1u : 0u;
+    }
+    info->roots_total = roots_all;
+    info->roots_kept = kept;
+    info->tasks = (uint32_t)out->tasks.size();
+    info->cliques = cliques;
+    info->plan_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  }
+  if (root_cliques) root_cliques->swap(ck);
+  return KCL_OK;
+}
+
 kcl_status sim_prepare(kcl_sim *s) {
   kcl_graph *g = s->g;
   const uint32_t n = g->n;
@@ -224,40 +263,10 @@ kcl_status sim_prepare(kcl_sim *s) {
   KCL_TRY(zero_u64(s->last.p, n, s->stream));
   KCL_TRY(zero_u64(s->V.p, n, s->stream));
   if (!s->trivial) {
-    // The plan is fixed per graph and k (rho only changes who wins):
-    //  1. cost per root from a root-credit (k-1)-pass (k >= 4);
-    //  2. a root-credit k-pass gives each root's k-clique count; roots with
-    //     none are dropped (exact: they contribute nothing to any pass);
-    //  3. tasks for the remaining roots, heavy ones split by cost.
-    const auto t0 = std::chrono::steady_clock::now();
-    std::vector<uint64_t> cost;
-    const std::vector<uint64_t> *cp = nullptr;
-    if (s->k >= 4) {
-      KCL_TRY(root_costs(g, s->k, s->stream, &cost));
-      cp = &cost;
-    }
     const uint32_t split = s->opts.split_work ? s->opts.split_work : (1u << 17);
-    PassPlan all;
-    KCL_TRY(build_plan(g->cd, s->k, cp, split, &all));
-    std::vector<uint64_t> ck;
-    KCL_TRY(root_counts(g, s->k, all, s->stream, &ck));
     PassPlan p;
-    KCL_TRY(build_plan(g->cd, s->k, cp, split, &p, &ck));
+    KCL_TRY(make_plan(g, s->k, split, s->stream, &p, nullptr, &s->info));
     KCL_TRY(upload_plan(p, &s->plan));
-    uint64_t cliques = 0;
-    uint32_t kept = 0;
-    for (uint32_t v = 0; v < n; ++v) {
-      cliques += ck[v];
-      kept += ck[v] ? 1u : 0u;
-    }
-    uint32_t roots_all = 0;
-    for (uint32_t v = 0; v < n; ++v) roots_all += (g->cd[v + 1] - g->cd[v] >= s->k - 1) ? 1u : 0u;
-    s->info.roots_total = roots_all;
-    s->info.roots_kept = kept;
-    s->info.tasks = (uint32_t)p.tasks.size();
-    s->info.cliques = cliques;
-    s->info.plan_ms =
-        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   }
   KCL_HIP(hipStreamSynchronize(s->stream));
   return KCL_OK;
