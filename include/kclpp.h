@@ -240,6 +240,44 @@ kcl_status kcl_trim_write(const kcl_trim *t, const char *ckdeg, const char *ckco
                           int64_t t0_unix);
 
 /* ---------------------------------------------------------------------
+ * Exact k-clique densest subgraph: the result of kCDensestMem
+ * (kCDensestMem.c:480-545), computed without storing every clique:
+ * Sim-kClist++ rounds on the GPU (1, 2, 4, ... iterations), the top PAVA
+ * block (kCDensestMem.c:329-364) as the candidate of density D = p/q, the
+ * D-core by k-clique degree as the region that must contain every densest
+ * subgraph, and a host max-flow over the region's cliques (Goldberg's
+ * network, kCDensestMem.c:434-467) that either certifies D - returning the
+ * maximal densest subgraph - or yields a denser candidate.  Rounds stop at
+ * max_iter iterations; regions whose clique count exceeds
+ * max_network_cliques are not sent to the max-flow.
+ * stat_path (nullable): the reference's stat_exact file (rows per round,
+ * then the final record).  nodes_out (nullable, room for n entries of the
+ * graph): input ids of the subgraph, ascending.
+ * ------------------------------------------------------------------- */
+typedef struct kcl_exact_opts {
+  uint32_t max_iter;             /* 0 = 65536 */
+  uint32_t reserved0;
+  uint64_t max_network_cliques;  /* 0 = 20,000,000 */
+  uint32_t reserved[6];
+} kcl_exact_opts;
+
+typedef struct kcl_exact_result {
+  uint32_t certified;      /* 1: optimality proved by the max-flow */
+  uint32_t num_iter;       /* Sim iterations run */
+  uint32_t n;              /* nodes of the (maximal) densest subgraph */
+  uint32_t m;              /* its edges */
+  uint64_t cliques;        /* its k-cliques: density = cliques / n exactly */
+  double density;
+  uint32_t max_flow_calls;
+  uint32_t region;         /* nodes of the last region */
+  double seconds;
+} kcl_exact_result;
+
+kcl_status kcl_exact_run(kcl_graph *g, uint32_t k, const kcl_exact_opts *opts,
+                         const char *stat_path, int64_t t0_unix, kcl_exact_result *res,
+                         uint32_t *nodes_out);
+
+/* ---------------------------------------------------------------------
  * Library information.
  * ------------------------------------------------------------------- */
 typedef struct kcl_device_info {

@@ -72,6 +72,17 @@ class _TrimInfo(C.Structure):
                 ("seconds_peel", C.c_double)]
 
 
+class _ExactOpts(C.Structure):
+    _fields_ = [("max_iter", C.c_uint32), ("reserved0", C.c_uint32),
+                ("max_network_cliques", C.c_uint64), ("reserved", C.c_uint32 * 6)]
+
+
+class _ExactResult(C.Structure):
+    _fields_ = [("certified", C.c_uint32), ("num_iter", C.c_uint32), ("n", C.c_uint32),
+                ("m", C.c_uint32), ("cliques", C.c_uint64), ("density", C.c_double),
+                ("max_flow_calls", C.c_uint32), ("region", C.c_uint32), ("seconds", C.c_double)]
+
+
 class _Record(C.Structure):
     _fields_ = [("T", C.c_uint32), ("n", C.c_uint32), ("m", C.c_uint32),
                 ("density", C.c_double), ("ub", C.c_double), ("sum", C.c_uint64),
@@ -92,7 +103,7 @@ EXPORTS = [
     "kcl_sim_destroy", "kcl_sim_step", "kcl_sim_counts", "kcl_sim_rho", "kcl_sim_set_rho",
     "kcl_sim_totals", "kcl_sim_last_pass", "kcl_sim_plan_info", "kcl_sim_find_densest", "kcl_cmpsync_run",
     "kcl_device_get_info", "kcl_version", "kcl_trim_create", "kcl_trim_destroy",
-    "kcl_trim_get_info", "kcl_trim_nodes", "kcl_trim_edges", "kcl_trim_write",
+    "kcl_trim_get_info", "kcl_trim_nodes", "kcl_trim_edges", "kcl_trim_write", "kcl_exact_run",
 ]
 
 
@@ -142,6 +153,8 @@ def lib() -> C.CDLL:
         "kcl_trim_edges": ([vp, vp], C.c_int),
         "kcl_trim_write": ([vp, C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p,
                             i64], C.c_int),
+        "kcl_exact_run": ([vp, u32, C.POINTER(_ExactOpts), C.c_char_p, i64,
+                           C.POINTER(_ExactResult), vp], C.c_int),
     }
     for name, (args, res) in sigs.items():
         f = getattr(L, name)
@@ -435,3 +448,19 @@ class Trim:
         enc = lambda p: p.encode() if p else None  # noqa: E731
         _check(lib().kcl_trim_write(self._h, enc(ckdeg), enc(ckcore), enc(ckdens), enc(trimmed),
                                     enc(stat), t0_unix))
+
+
+def exact(graph: Graph, k: int, max_iter: int = 0, max_network_cliques: int = 0,
+          stat_path: str | None = None, t0_unix: int = 0):
+    """Exact k-clique densest subgraph (kCDensestMem's result).  Returns
+    (result dict, input ids of the maximal densest subgraph)."""
+    o = _ExactOpts()
+    o.max_iter = max_iter
+    o.max_network_cliques = max_network_cliques
+    r = _ExactResult()
+    nodes = np.zeros(max(graph.n, 1), np.uint32)
+    _check(lib().kcl_exact_run(graph.handle, k, C.byref(o),
+                               stat_path.encode() if stat_path else None, t0_unix, C.byref(r),
+                               _ptr(nodes)))
+    res = {f: getattr(r, f) for f, _ in _ExactResult._fields_}
+    return res, nodes[:r.n].copy()

@@ -35,10 +35,11 @@ HIP_FLAGS = [
     "-Wall", "-Wno-unused-function", "-I", INCLUDE, "-I", CSRC,
 ]
 
-SOURCES = ["host_io.cpp", "host_order.cpp", "graph.hip", "pass.hip", "sim.hip", "trim.cpp"]
+SOURCES = ["host_io.cpp", "host_order.cpp", "graph.hip", "pass.hip", "sim.hip", "trim.cpp", "exact.cpp"]
 CLIS = {
     "kCDensestCmpSync": "cli_cmpsync.c",
     "kClistCoreTrim": "cli_trim.c",
+    "kCDensestMem": "cli_exact.c",
 }
 
 

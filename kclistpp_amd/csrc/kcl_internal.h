@@ -123,6 +123,10 @@ kcl_status upload_plan(const PassPlan &plan, DevicePlan *dev);
 kcl_status make_plan(kcl_graph *g, uint32_t k, uint32_t split, hipStream_t st, PassPlan *out,
                      std::vector<uint64_t> *root_cliques, kcl_plan_info *info);
 
+kcl_status sim_step(kcl_sim *s, uint32_t t);
+kcl_status sim_level_pass(kcl_sim *s, bool full_key, std::vector<unsigned long long> &tent,
+                          std::vector<uint32_t> &reo, std::vector<double> &rho);
+
 // Largest k with compiled enumeration kernels.
 uint32_t max_pass_k();
 

@@ -99,11 +99,14 @@ __global__ void rho_update_kernel(uint32_t n, uint32_t t, double *__restrict__ r
 // Sort key of CDF_NodeCmp (kCDensestCmpSync.c:334-340): (unsigned long
 // long)rho descending; complemented so an ascending stable radix sort of
 // (key, id) pairs gives floor(rho) descending, ties by id ascending.
+// full != 0: the whole double descending (kCDensestMem.c:314-319; rho >= +0.0
+// so its IEEE bits order like the values).
 __global__ void level_key_kernel(uint32_t n, const double *__restrict__ rho,
                                  unsigned long long *__restrict__ keys,
-                                 uint32_t *__restrict__ iota) {
+                                 uint32_t *__restrict__ iota, uint32_t full) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    keys[i] = ~(unsigned long long)rho[i];
+    keys[i] = full ? ~(unsigned long long)__double_as_longlong(rho[i])
+                   : ~(unsigned long long)rho[i];
     iota[i] = i;
   }
 }
@@ -310,17 +313,14 @@ kcl_status sim_step(kcl_sim *s, uint32_t t) {
   return KCL_OK;
 }
 
-kcl_status sim_find_densest(kcl_sim *s, kcl_densest_info *out, uint32_t *h_reordered,
-                            uint32_t *h_level, uint64_t *h_tent) {
+// Level order (floor(rho) or full rho, descending, ties by id) and the
+// PAVA-preprocess pass: tent[level] = cliques whose last-level member sits
+// at that level.  Host copies of tent, reordered and rho.
+kcl_status sim_level_pass(kcl_sim *s, bool full_key, std::vector<unsigned long long> &tent,
+                          std::vector<uint32_t> &reo, std::vector<double> &rho) {
   kcl_graph *g = s->g;
   const uint32_t n = g->n;
   hipStream_t st = s->stream;
-  out->n = 0;
-  out->m = 0;
-  out->density = -1;
-  out->ub = 0;
-  out->sum = 0;
-  if (n == 0) return KCL_OK;  // the reference reads an unset n here
   if (!s->level.p) {
     KCL_TRY(s->level.alloc(n));
     KCL_TRY(s->reordered.alloc(n));
@@ -334,8 +334,8 @@ kcl_status sim_find_densest(kcl_sim *s, kcl_densest_info *out, uint32_t *h_reord
                                       64, st));
     KCL_TRY(s->sort_tmp.alloc(s->sort_tmp_bytes));
   }
-  // Level order.
-  level_key_kernel<<<grid_for(n, 256), 256, 0, st>>>(n, s->rho.p, s->sort_keys_in.p, s->iota.p);
+  level_key_kernel<<<grid_for(n, 256), 256, 0, st>>>(n, s->rho.p, s->sort_keys_in.p, s->iota.p,
+                                                     full_key ? 1u : 0u);
   KCL_HIP(hipGetLastError());
   size_t tb = s->sort_tmp_bytes;
   KCL_HIP(rocprim::radix_sort_pairs(s->sort_tmp.p, tb, s->sort_keys_in.p, s->sort_keys_out.p,
@@ -349,9 +349,9 @@ kcl_status sim_find_densest(kcl_sim *s, kcl_densest_info *out, uint32_t *h_reord
   tent_scatter_kernel<<<grid_for(n, 256), 256, 0, st>>>(n, s->level.p, s->cnt.p, s->tent.p,
                                                         s->stats.p);
   KCL_HIP(hipGetLastError());
-  std::vector<unsigned long long> tent(n);
-  std::vector<uint32_t> reo(n);
-  std::vector<double> rho(n);
+  tent.resize(n);
+  reo.resize(n);
+  rho.resize(n);
   unsigned long long stv[2] = {0, 0};
   KCL_HIP(hipMemcpyAsync(tent.data(), s->tent.p, (size_t)n * 8, hipMemcpyDeviceToHost, st));
   KCL_HIP(hipMemcpyAsync(reo.data(), s->reordered.p, (size_t)n * 4, hipMemcpyDeviceToHost, st));
@@ -359,6 +359,24 @@ kcl_status sim_find_densest(kcl_sim *s, kcl_densest_info *out, uint32_t *h_reord
   KCL_HIP(hipMemcpyAsync(stv, s->stats.p, 16, hipMemcpyDeviceToHost, st));
   KCL_HIP(hipStreamSynchronize(st));
   if (stv[1]) return fail(KCL_E_OVERFLOW, "a level received >= 2^53 cliques");
+  return KCL_OK;
+}
+
+kcl_status sim_find_densest(kcl_sim *s, kcl_densest_info *out, uint32_t *h_reordered,
+                            uint32_t *h_level, uint64_t *h_tent) {
+  kcl_graph *g = s->g;
+  const uint32_t n = g->n;
+  hipStream_t st = s->stream;
+  out->n = 0;
+  out->m = 0;
+  out->density = -1;
+  out->ub = 0;
+  out->sum = 0;
+  if (n == 0) return KCL_OK;  // the reference reads an unset n here
+  std::vector<unsigned long long> tent;
+  std::vector<uint32_t> reo;
+  std::vector<double> rho;
+  KCL_TRY(sim_level_pass(s, false, tent, reo, rho));
 
   // Densest prefix (kCDensestCmpSync.c:354-361): the running sum is a sum
   // of integers below 2^53, so the double is exact and matches.
