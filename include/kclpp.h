@@ -160,6 +160,14 @@ typedef struct kcl_plan_info {
 
 kcl_status kcl_sim_plan_info(kcl_sim *s, kcl_plan_info *info);
 
+/* One Frank-Wolfe pass at the current rho with per-task timing, leaving the
+ * state unchanged.  *ntasks: in = room in the arrays, out = tasks of the
+ * plan (all arrays NULL: size query).  Per task (plan order, nullable):
+ * root node, part, parts of that root, shader-clock cycles until the root's
+ * rows were built, cycles of the whole task. */
+kcl_status kcl_sim_profile_pass(kcl_sim *s, uint32_t *ntasks, uint32_t *root, uint32_t *part,
+                                uint32_t *nparts, uint64_t *build_cycles, uint64_t *total_cycles);
+
 typedef struct kcl_densest_info {
   uint32_t n;       /* nodes of the densest prefix */
   uint32_t m;       /* edges inside it */

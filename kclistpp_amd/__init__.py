@@ -101,7 +101,7 @@ EXPORTS = [
     "kcl_graph_create", "kcl_graph_destroy", "kcl_graph_get_info", "kcl_graph_rank",
     "kcl_graph_csr", "kcl_graph_edges", "kcl_count_edges", "kcl_clique_enum", "kcl_sim_create",
     "kcl_sim_destroy", "kcl_sim_step", "kcl_sim_counts", "kcl_sim_rho", "kcl_sim_set_rho",
-    "kcl_sim_totals", "kcl_sim_last_pass", "kcl_sim_plan_info", "kcl_sim_find_densest", "kcl_cmpsync_run",
+    "kcl_sim_totals", "kcl_sim_last_pass", "kcl_sim_plan_info", "kcl_sim_profile_pass", "kcl_sim_find_densest", "kcl_cmpsync_run",
     "kcl_device_get_info", "kcl_version", "kcl_trim_create", "kcl_trim_destroy",
     "kcl_trim_get_info", "kcl_trim_nodes", "kcl_trim_edges", "kcl_trim_write", "kcl_exact_run",
 ]
@@ -142,6 +142,7 @@ def lib() -> C.CDLL:
         "kcl_sim_last_pass": ([vp, C.POINTER(u64), C.POINTER(C.c_double),
                                C.POINTER(C.c_double)], C.c_int),
         "kcl_sim_plan_info": ([vp, C.POINTER(_PlanInfo)], C.c_int),
+        "kcl_sim_profile_pass": ([vp, C.POINTER(u32), vp, vp, vp, vp, vp], C.c_int),
         "kcl_sim_find_densest": ([vp, C.POINTER(_Densest), vp, vp, vp], C.c_int),
         "kcl_cmpsync_run": ([vp, u32, u32, C.POINTER(_SimOpts), C.c_char_p, i64,
                              C.POINTER(_Record), C.POINTER(u32)], C.c_int),
@@ -366,6 +367,20 @@ class Sim:
         _check(lib().kcl_sim_plan_info(self._h, C.byref(p)))
         return {"roots_total": p.roots_total, "roots_kept": p.roots_kept, "tasks": p.tasks,
                 "cliques": int(p.cliques), "plan_ms": p.plan_ms}
+
+    def profile_pass(self) -> dict:
+        """One pass with per-task shader-clock timing (state unchanged)."""
+        nt = C.c_uint32(0)
+        _check(lib().kcl_sim_profile_pass(self._h, C.byref(nt), None, None, None, None, None))
+        n = nt.value
+        out = {"root": np.zeros(max(n, 1), np.uint32), "part": np.zeros(max(n, 1), np.uint32),
+               "nparts": np.zeros(max(n, 1), np.uint32),
+               "build_cycles": np.zeros(max(n, 1), np.uint64),
+               "total_cycles": np.zeros(max(n, 1), np.uint64)}
+        _check(lib().kcl_sim_profile_pass(self._h, C.byref(nt), _ptr(out["root"]),
+                                          _ptr(out["part"]), _ptr(out["nparts"]),
+                                          _ptr(out["build_cycles"]), _ptr(out["total_cycles"])))
+        return {k: v[:n] for k, v in out.items()}
 
     def find_densest(self, want_vectors: bool = False):
         d = _Densest()

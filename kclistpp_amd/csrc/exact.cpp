@@ -230,7 +230,7 @@ kcl_status alive_degrees(const kcl_graph *g, uint32_t k, const std::vector<uint8
         st = fail(KCL_E_HIP, "hipMemset: %s", hipGetErrorString(e));
         break;
       }
-      if ((st = launch_pass(sub->d_cd.p, sub->d_adj.p, (const uint64_t *)key.p, k, kCreditAll, dp,
+      if ((st = launch_pass(sub, (const uint64_t *)key.p, k, kCreditAll, dp,
                             cnt.p, sub->stream)) != KCL_OK)
         break;
       e = hipMemcpyAsync(deg->data(), cnt.p, (size_t)sub->n * 8, hipMemcpyDeviceToHost,

@@ -80,6 +80,22 @@ __global__ void count_edges_kernel(const uint32_t *__restrict__ cd,
   if ((threadIdx.x & 63) == 0 && local) atomicAdd(out, local);
 }
 
+// Dense bitmap of the edges among nodes < h: one thread per row s < h sets
+// (s, t) and (t, s) for its out-neighbours t (t < s).
+__global__ void core_bits_kernel(const uint32_t *__restrict__ cd, const uint32_t *__restrict__ adj,
+                                 uint32_t rows, uint32_t width,
+                                 unsigned long long *__restrict__ bits) {
+  const uint32_t stride = width >> 6;
+  for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < rows;
+       s += gridDim.x * blockDim.x) {
+    for (uint32_t j = cd[s]; j < cd[s + 1]; ++j) {
+      const uint32_t t = adj[j];
+      atomicOr(&bits[(size_t)s * stride + (t >> 6)], 1ull << (t & 63));
+      atomicOr(&bits[(size_t)t * stride + (s >> 6)], 1ull << (s & 63));
+    }
+  }
+}
+
 uint32_t grid_for(uint64_t work, uint32_t block) {
   uint64_t g = (work + block - 1) / block;
   if (g > 65536) g = 65536;
@@ -153,6 +169,18 @@ static kcl_status build_device_csr(kcl_graph *g, const uint32_t *pairs) {
                          st));
   KCL_HIP(hipStreamSynchronize(st));
   g->core = core;
+  // Core bitmap: nodes [0, h) with h = min(kCoreMax, n rounded up to 64).
+  const uint32_t h = std::min<uint32_t>(kCoreMax, (g->n + 63) & ~63u);
+  if (h > 0) {
+    KCL_TRY(g->d_core.alloc((size_t)h * (h / 64)));
+    KCL_HIP(hipMemsetAsync(g->d_core.p, 0, (size_t)h * (h / 64) * 8, st));
+    const uint32_t rows = std::min(h, g->n);
+    core_bits_kernel<<<grid_for(rows, 256), 256, 0, st>>>(g->d_cd.p, g->d_adj.p, rows, h,
+                                                          g->d_core.p);
+    KCL_HIP(hipGetLastError());
+    KCL_HIP(hipStreamSynchronize(st));
+    g->core_h = h;
+  }
   return KCL_OK;
 }
 

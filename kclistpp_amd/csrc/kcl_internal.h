@@ -90,6 +90,7 @@ struct PassPlan {
 };
 
 struct DevicePlan {
+  std::vector<Task> host_tasks;  // same order as tasks
   DevBuf<Task> tasks;
   DevBuf<uint32_t> tickets;  // one queue head per class
   uint32_t class_begin[kNumClasses + 1] = {};
@@ -107,15 +108,29 @@ kcl_status device_props(DeviceProps *out);
 // Enumeration pass over a plan: cnt[x] += credits.  key[x] orders members
 // (smaller wins, ties to the smaller id).  All launches go to `stream`;
 // the ms of the enumeration kernels is recorded between ev0 and ev1.
-kcl_status launch_pass(const uint32_t *d_cd, const uint32_t *d_adj, const uint64_t *d_key,
-                       uint32_t k, PassMode mode, DevicePlan &plan, unsigned long long *d_cnt,
-                       hipStream_t stream);
+// prof (nullable): 2 u64 per task, shader-clock cycles until the rows are
+// built and of the whole task (the tasks' order is the plan's).
+kcl_status launch_pass(const kcl_graph *g, const uint64_t *d_key, uint32_t k, PassMode mode,
+                       DevicePlan &plan, unsigned long long *d_cnt, hipStream_t stream,
+                       unsigned long long *prof = nullptr);
 
-// Work-unit estimate per root from a root-credit pass of (k-1)-cliques.
+// Largest core bitmap side (ids below it get a dense adjacency bitmap).
+constexpr uint32_t kCoreMax = 16384;
+
+// Work estimate per root: counting work (steps x row words) and build work
+// (adjacency entries scanned).  A split part counts at least kBuildShare
+// times its root's build work.
+struct RootCost {
+  std::vector<uint64_t> work;
+  std::vector<uint64_t> build;
+};
+constexpr uint64_t kBuildShare = 8;
+
+// Tasks for every root with d+(s) >= k-1 (rc nullable: d^2 estimate).
 // keep (nullable): only roots with keep[s] != 0 get tasks.
-kcl_status build_plan(const std::vector<uint32_t> &cd, uint32_t k,
-                      const std::vector<uint64_t> *root_cost, uint32_t split_work,
-                      PassPlan *plan, const std::vector<uint64_t> *keep = nullptr);
+kcl_status build_plan(const std::vector<uint32_t> &cd, uint32_t k, const RootCost *rc,
+                      uint32_t split_work, PassPlan *plan,
+                      const std::vector<uint64_t> *keep = nullptr);
 kcl_status upload_plan(const PassPlan &plan, DevicePlan *dev);
 
 // Plan for (graph, k): cost pass, root filter, split tasks.  root_cliques
@@ -149,5 +164,9 @@ struct kcl_graph {
   std::vector<uint32_t> adj;   // m (host copy)
   kcl::DevBuf<uint32_t> d_cd;
   kcl::DevBuf<uint32_t> d_adj;
+  // Dense adjacency bitmap of the nodes with id < core_h (the most core
+  // part of the degeneracy order): row v, bit u set iff {u, v} is an edge.
+  kcl::DevBuf<unsigned long long> d_core;
+  uint32_t core_h = 0;
   hipStream_t stream = nullptr;
 };

@@ -37,9 +37,9 @@ size_t lds_bytes(uint32_t wclass) { return pass_table().lds[wclass]; }
 
 uint32_t max_pass_k() { return (uint32_t)kMaxTemplK; }
 
-kcl_status launch_pass(const uint32_t *d_cd, const uint32_t *d_adj, const uint64_t *d_key,
-                       uint32_t k, PassMode mode, DevicePlan &plan, unsigned long long *d_cnt,
-                       hipStream_t stream) {
+kcl_status launch_pass(const kcl_graph *g, const uint64_t *d_key, uint32_t k, PassMode mode,
+                       DevicePlan &plan, unsigned long long *d_cnt, hipStream_t stream,
+                       unsigned long long *prof) {
   DeviceProps props;
   KCL_TRY(device_props(&props));
   KCL_HIP(hipMemsetAsync(plan.tickets.p, 0, kNumClasses * sizeof(uint32_t), stream));
@@ -55,17 +55,19 @@ kcl_status launch_pass(const uint32_t *d_cd, const uint32_t *d_adj, const uint64
     if (per_cu > 32 / waves) per_cu = 32 / waves;
     uint64_t blocks = (uint64_t)props.cu_count * per_cu;
     if (blocks > count) blocks = count;
-    hipLaunchKernelGGL(fn, dim3((uint32_t)blocks), dim3(64 * waves), 0, stream, d_cd, d_adj, d_key,
+    hipLaunchKernelGGL(fn, dim3((uint32_t)blocks), dim3(64 * waves), 0, stream,
+                       (const uint32_t *)g->d_cd.p, (const uint32_t *)g->d_adj.p, d_key,
+                       (const unsigned long long *)(g->core_h ? g->d_core.p : nullptr), g->core_h,
                        plan.tasks.p + plan.class_begin[c], count, plan.tickets.p + c,
-                       (uint32_t)mode, d_cnt);
+                       (uint32_t)mode, d_cnt,
+                       prof ? prof + 2 * (size_t)plan.class_begin[c] : nullptr);
     KCL_HIP(hipGetLastError());
   }
   return KCL_OK;
 }
 
-kcl_status build_plan(const std::vector<uint32_t> &cd, uint32_t k,
-                      const std::vector<uint64_t> *root_cost, uint32_t split_work,
-                      PassPlan *plan, const std::vector<uint64_t> *keep) {
+kcl_status build_plan(const std::vector<uint32_t> &cd, uint32_t k, const RootCost *rc,
+                      uint32_t split_work, PassPlan *plan, const std::vector<uint64_t> *keep) {
   const uint32_t n = cd.empty() ? 0 : (uint32_t)(cd.size() - 1);
   struct Entry {
     uint64_t cost;
@@ -85,10 +87,15 @@ kcl_status build_plan(const std::vector<uint32_t> &cd, uint32_t k,
     const uint32_t words = (d + 63) / 64;
     uint32_t c = 0;
     while (kClassWords[c] < words) ++c;  // smallest class that holds d members
-    uint64_t cost = root_cost ? (*root_cost)[s] : (uint64_t)d * d;
+    const uint64_t work = rc ? rc->work[s] : (uint64_t)d * d;
+    const uint64_t build = rc ? rc->build[s] : (uint64_t)d;
+    const uint64_t cost = work + build;
     uint32_t np = 1;
-    if (split_work && cost > split_work) {
-      uint64_t want = (cost + split_work - 1) / split_work;
+    // Each part rebuilds the root's rows, so a part takes at least
+    // kBuildShare times the build's work units of counting.
+    const uint64_t part_work = std::max<uint64_t>(split_work, kBuildShare * build);
+    if (split_work && work > part_work) {
+      uint64_t want = (work + part_work - 1) / part_work;
       const uint64_t items = (k >= 5) ? (uint64_t)d * (d - 1) / 2 : d;
       if (want > items) want = items;
       if (want > 4096) want = 4096;
@@ -116,6 +123,7 @@ kcl_status build_plan(const std::vector<uint32_t> &cd, uint32_t k,
 }
 
 kcl_status upload_plan(const PassPlan &plan, DevicePlan *dev) {
+  dev->host_tasks = plan.tasks;
   KCL_TRY(dev->tasks.alloc(plan.tasks.size()));
   KCL_TRY(dev->tickets.alloc(kNumClasses));
   if (!plan.tasks.empty())

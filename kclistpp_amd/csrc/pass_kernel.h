@@ -174,11 +174,14 @@ template <int W, int K, int NW, bool ALL>
 __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restrict__ cd,
                                                        const uint32_t *__restrict__ adj,
                                                        const uint64_t *__restrict__ key,
+                                                       const unsigned long long *__restrict__ core,
+                                                       uint32_t core_h,
                                                        const Task *__restrict__ tasks,
                                                        uint32_t ntasks,
                                                        uint32_t *__restrict__ ticket,
                                                        uint32_t mode,
-                                                       unsigned long long *__restrict__ cnt) {
+                                                       unsigned long long *__restrict__ cnt,
+                                                       unsigned long long *__restrict__ prof) {
   static_assert(K >= 2, "k >= 2");
   constexpr uint32_t NT = 64 * NW;  // threads of the workgroup sharing one root
   __shared__ Lds<W> sh;
@@ -191,6 +194,9 @@ __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restric
     __syncthreads();
     const uint32_t ti = sh.task;
     if (ti >= ntasks) break;  // same value in every thread of the workgroup
+    // Optional per-task timing (shader clock): prof[2 ti] = cycles until the
+    // rows are built, prof[2 ti + 1] = cycles of the whole task.
+    const unsigned long long t_start = prof ? __builtin_amdgcn_s_memtime() : 0ull;
     const Task tk = tasks[ti];
     const uint32_t s = tk.s;
     const uint32_t part = tk.part_np & 0xffffu, np = tk.part_np >> 16;
@@ -232,34 +238,97 @@ __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restric
         }
       }
     } else {
-      // Oriented rows from the members' adjacency slices.
+      // Oriented rows.  Members are ascending, so those inside the core
+      // bitmap (id < core_h) are a prefix of h members; the edges among them
+      // come from h(h-1)/2 bit tests when that is cheaper than scanning their
+      // adjacency slices.  The other members' slices are scanned: an edge
+      // (x, y) with y < x has x outside the core unless both ends are inside.
       if (tid < 64) wave_scan(sh.pre + 1, d, lane);
       __syncthreads();
       const uint32_t total = sh.pre[d];
-      uint32_t cur = 0;
-      for (uint32_t e = tid; e < total; e += NT) {
-        uint32_t lo = cur, hi = d;  // member owning element e: pre[i] <= e < pre[i+1]
-        while (hi - lo > 1) {
-          const uint32_t mid = (lo + hi) >> 1;
-          if (sh.pre[mid] <= e) lo = mid;
-          else hi = mid;
-        }
-        cur = lo;
-        const uint32_t i = lo;
-        const uint32_t y = adj[sh.beg[i] + (e - sh.pre[i])];
-        uint32_t a = 0, b = i;  // y among members [0, i) (all smaller ids)
+      uint32_t h = 0;
+      if (core) {
+        uint32_t a = 0, b = d;
         while (a < b) {
           const uint32_t mid = (a + b) >> 1;
-          if (sh.node[mid] < y) a = mid + 1;
+          if (sh.node[mid] < core_h) a = mid + 1;
           else b = mid;
         }
-        if (a < i && sh.node[a] == y) {
-          const bool a_first = sh.key[a] <= sh.key[i];  // ties: smaller id (a) first
-          const uint32_t from = a_first ? a : i, to = a_first ? i : a;
-          atomicOr(&sh.rows[from * W + (to >> 6)], 1ull << (to & 63));
+        h = a;
+        if ((uint64_t)h * (h - 1) / 2 >= (uint64_t)sh.pre[h]) h = 0;  // slices are cheaper
+      }
+      // kBuildBatch loads per thread are issued before any is used, so their
+      // latencies overlap instead of adding up.
+      constexpr uint32_t kBuildBatch = 8;
+      auto set_edge = [&](uint32_t a, uint32_t i) {  // members a < i are adjacent
+        const bool a_first = sh.key[a] <= sh.key[i];  // ties: smaller id (a) first
+        const uint32_t from = a_first ? a : i, to = a_first ? i : a;
+        atomicOr(&sh.rows[from * W + (to >> 6)], 1ull << (to & 63));
+      };
+      if (h > 1) {
+        const uint32_t pairs = h * (h - 1) / 2;
+        const uint32_t stride = core_h >> 6;  // words per bitmap row
+        for (uint32_t p0 = tid; p0 < pairs; p0 += NT * kBuildBatch) {
+          unsigned long long wv[kBuildBatch];
+          uint32_t iv[kBuildBatch], jv[kBuildBatch];
+#pragma unroll
+          for (uint32_t b = 0; b < kBuildBatch; ++b) {
+            const uint32_t pi = p0 + b * NT;
+            iv[b] = UINT32_MAX;
+            if (pi < pairs) {
+              // pair index pi = i(i-1)/2 + j, 0 <= j < i
+              uint32_t i = (uint32_t)((1.0f + sqrtf(1.0f + 8.0f * (float)pi)) * 0.5f);
+              while ((uint64_t)i * (i - 1) / 2 > pi) --i;
+              while ((uint64_t)(i + 1) * i / 2 <= pi) ++i;
+              const uint32_t j = pi - i * (i - 1) / 2;
+              iv[b] = i;
+              jv[b] = j;
+              const uint32_t y = sh.node[j];
+              wv[b] = core[(size_t)sh.node[i] * stride + (y >> 6)];
+            }
+          }
+#pragma unroll
+          for (uint32_t b = 0; b < kBuildBatch; ++b) {
+            if (iv[b] == UINT32_MAX) continue;
+            if ((wv[b] >> (sh.node[jv[b]] & 63)) & 1ull) set_edge(jv[b], iv[b]);
+          }
+        }
+      }
+      uint32_t cur = h;
+      for (uint32_t e0 = sh.pre[h] + tid; e0 < total; e0 += NT * kBuildBatch) {
+        uint32_t yv[kBuildBatch], iv[kBuildBatch];
+#pragma unroll
+        for (uint32_t b = 0; b < kBuildBatch; ++b) {
+          const uint32_t e = e0 + b * NT;
+          iv[b] = UINT32_MAX;
+          if (e < total) {
+            uint32_t lo = cur, hi = d;  // member owning element e: pre[i] <= e < pre[i+1]
+            while (hi - lo > 1) {
+              const uint32_t mid = (lo + hi) >> 1;
+              if (sh.pre[mid] <= e) lo = mid;
+              else hi = mid;
+            }
+            cur = lo;
+            iv[b] = lo;
+            yv[b] = adj[sh.beg[lo] + (e - sh.pre[lo])];
+          }
+        }
+#pragma unroll
+        for (uint32_t b = 0; b < kBuildBatch; ++b) {
+          const uint32_t i = iv[b];
+          if (i == UINT32_MAX) continue;
+          const uint32_t y = yv[b];
+          uint32_t a = 0, c = i;  // y among members [0, i) (all smaller ids)
+          while (a < c) {
+            const uint32_t mid = (a + c) >> 1;
+            if (sh.node[mid] < y) a = mid + 1;
+            else c = mid;
+          }
+          if (a < i && sh.node[a] == y) set_edge(a, i);
         }
       }
       __syncthreads();
+      if (prof && tid == 0) prof[2 * (size_t)ti] = __builtin_amdgcn_s_memtime() - t_start;
 
       if constexpr (K == 3) {
         const uint32_t lo = (uint32_t)((uint64_t)part * d / np);
@@ -410,6 +479,7 @@ __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restric
     if (lane == 0 && root_acc) atomicAdd(&sh.root_sum, root_acc);
     __syncthreads();
     if (tid == 0 && sh.root_sum) atomicAdd(&cnt[s], sh.root_sum);
+    if (prof && tid == 0) prof[2 * (size_t)ti + 1] = __builtin_amdgcn_s_memtime() - t_start;
     for (uint32_t i = tid; i < d; i += NT) {
       const unsigned long long c = sh.cred[i];
       if (c) atomicAdd(&cnt[sh.node[i]], c);
@@ -420,8 +490,9 @@ __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restric
 
 }  // namespace
 
-using PassFn = void (*)(const uint32_t *, const uint32_t *, const uint64_t *, const Task *,
-                        uint32_t, uint32_t *, uint32_t, unsigned long long *);
+using PassFn = void (*)(const uint32_t *, const uint32_t *, const uint64_t *,
+                        const unsigned long long *, uint32_t, const Task *, uint32_t, uint32_t *,
+                        uint32_t, unsigned long long *, unsigned long long *);
 
 constexpr int kMaxTemplK = 10;  // k handled by compiled kernels: 2..kMaxTemplK
 

@@ -179,7 +179,7 @@ kcl_status root_counts(kcl_graph *g, uint32_t k, const PassPlan &plan, hipStream
   KCL_TRY(key.alloc(g->n));
   KCL_TRY(zero_u64(c.p, g->n, st));
   KCL_TRY(zero_u64(key.p, g->n, st));
-  KCL_TRY(launch_pass(g->d_cd.p, g->d_adj.p, (const uint64_t *)key.p, k, kCreditRoot, dp, c.p,
+  KCL_TRY(launch_pass(g, (const uint64_t *)key.p, k, kCreditRoot, dp, c.p,
                       st));
   KCL_HIP(hipMemcpyAsync(out->data(), c.p, (size_t)g->n * sizeof(uint64_t),
                          hipMemcpyDeviceToHost, st));
@@ -191,17 +191,19 @@ kcl_status root_counts(kcl_graph *g, uint32_t k, const PassPlan &plan, hipStream
 // root, counted by a root-credit (k-1)-pass) times the row width, plus the
 // adjacency entries its build scans (so light roots keep a weight and split
 // parts amortise their rebuild).
-kcl_status root_costs(kcl_graph *g, uint32_t k, hipStream_t st, std::vector<uint64_t> *out) {
+kcl_status root_costs(kcl_graph *g, uint32_t k, hipStream_t st, RootCost *out) {
   PassPlan p;
   KCL_TRY(build_plan(g->cd, k - 1, nullptr, 0, &p));
-  KCL_TRY(root_counts(g, k - 1, p, st, out));
+  KCL_TRY(root_counts(g, k - 1, p, st, &out->work));
+  out->build.assign(g->n, 0);
   for (uint32_t s = 0; s < g->n; ++s) {
     uint64_t b = 0;
     for (uint32_t j = g->cd[s]; j < g->cd[s + 1]; ++j) {
       const uint32_t x = g->adj[j];
       b += g->cd[x + 1] - g->cd[x];
     }
-    (*out)[s] = (*out)[s] * ((g->cd[s + 1] - g->cd[s] + 63) / 64) + b;
+    out->build[s] = b;
+    out->work[s] *= (g->cd[s + 1] - g->cd[s] + 63) / 64;
   }
   return KCL_OK;
 }
@@ -217,8 +219,8 @@ kcl_status make_plan(kcl_graph *g, uint32_t k, uint32_t split, hipStream_t st, P
                      std::vector<uint64_t> *root_cliques, kcl_plan_info *info) {
   const auto t0 = std::chrono::steady_clock::now();
   const uint32_t n = g->n;
-  std::vector<uint64_t> cost;
-  const std::vector<uint64_t> *cp = nullptr;
+  RootCost cost;
+  const RootCost *cp = nullptr;
   if (k >= 4) {
     KCL_TRY(root_costs(g, k, st, &cost));
     cp = &cost;
@@ -279,7 +281,7 @@ kcl_status sim_prepare(kcl_sim *s) {
 static kcl_status sim_pass(kcl_sim *s, const uint64_t *d_key, PassMode mode) {
   KCL_HIP(hipEventRecord(s->ev[0], s->stream));
   if (!s->trivial)
-    KCL_TRY(launch_pass(s->g->d_cd.p, s->g->d_adj.p, d_key, s->k, mode, s->plan, s->cnt.p,
+    KCL_TRY(launch_pass(s->g, d_key, s->k, mode, s->plan, s->cnt.p,
                         s->stream));
   KCL_HIP(hipEventRecord(s->ev[1], s->stream));
   return KCL_OK;
@@ -518,6 +520,36 @@ extern "C" kcl_status kcl_sim_last_pass(kcl_sim *s, uint64_t *visits, double *ke
   if (visits) *visits = s->last_visits;
   if (kernel_ms) *kernel_ms = s->last_kernel_ms;
   if (pass_ms) *pass_ms = s->last_pass_ms;
+  return KCL_OK;
+}
+
+extern "C" kcl_status kcl_sim_profile_pass(kcl_sim *s, uint32_t *ntasks, uint32_t *root,
+                                           uint32_t *part, uint32_t *nparts, uint64_t *build_cycles,
+                                           uint64_t *total_cycles) {
+  if (!s || !ntasks) return fail(KCL_E_INVALID_ARG, "NULL argument");
+  const uint32_t nt = (uint32_t)s->plan.host_tasks.size();
+  const uint32_t cap = *ntasks;
+  *ntasks = nt;
+  if (!root && !part && !nparts && !build_cycles && !total_cycles) return KCL_OK;  // size query
+  if (cap < nt) return fail(KCL_E_INVALID_ARG, "room for %u tasks, the plan has %u", cap, nt);
+  if (s->trivial || nt == 0) return KCL_OK;
+  kcl::DevBuf<unsigned long long> prof;
+  KCL_TRY(prof.alloc(2 * (size_t)nt));
+  KCL_HIP(hipMemsetAsync(prof.p, 0, 2 * (size_t)nt * 8, s->stream));
+  KCL_TRY(kcl::launch_pass(s->g, (const uint64_t *)s->rho.p, s->k,
+                           kcl::kCreditWinner, s->plan, s->cnt.p, s->stream, prof.p));
+  std::vector<unsigned long long> h(2 * (size_t)nt);
+  KCL_HIP(hipMemcpyAsync(h.data(), prof.p, h.size() * 8, hipMemcpyDeviceToHost, s->stream));
+  KCL_TRY(kcl::zero_u64(s->cnt.p, s->g->n, s->stream));  // the pass leaves no credits behind
+  KCL_HIP(hipStreamSynchronize(s->stream));
+  for (uint32_t i = 0; i < nt; ++i) {
+    const kcl::Task &t = s->plan.host_tasks[i];
+    if (root) root[i] = t.s;
+    if (part) part[i] = t.part_np & 0xffffu;
+    if (nparts) nparts[i] = t.part_np >> 16;
+    if (build_cycles) build_cycles[i] = h[2 * (size_t)i];
+    if (total_cycles) total_cycles[i] = h[2 * (size_t)i + 1];
+  }
   return KCL_OK;
 }
 

@@ -236,7 +236,7 @@ kcl_status trim_run(kcl_trim *T, const uint32_t *pairs, uint64_t m, uint32_t n_i
         st = fail(KCL_E_HIP, "hipMemset: %s", hipGetErrorString(e));
         break;
       }
-      st = launch_pass(g->d_cd.p, g->d_adj.p, (const uint64_t *)key.p, kmax, kCreditAll, dp, cnt.p,
+      st = launch_pass(g, (const uint64_t *)key.p, kmax, kCreditAll, dp, cnt.p,
                        g->stream);
       if (st != KCL_OK) break;
       e = hipMemcpyAsync(T->nck.data(), cnt.p, (size_t)g->n * 8, hipMemcpyDeviceToHost, g->stream);
