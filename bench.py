@@ -33,6 +33,8 @@ WORKLOADS = {
              "Chung-Lu n=1e6 m=1.6e7 gamma=2.3 w_max=2e4 + 150-vertex community p=0.8, k=5"),
     "er20k": ("er20k", 1, 1, 4, "Erdos-Renyi n=2e4 m=1e5 + planted K30, k=4"),
     "cl1m_d16": ("cl1m", 1, 16, 5, "Chung-Lu cl1m scaled down 16x, k=5 (quick check)"),
+    "cl4m": ("cl4m", 1, 1, 6,
+             "Chung-Lu n=4e6 m=6e7 gamma=2.1 w_max=4e4 + 200-vertex near-clique p=0.9, k=6"),
 }
 
 
@@ -40,18 +42,29 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes(cd: np.ndarray, adj: np.ndarray, k: int) -> tuple[int, int]:
+def algorithmic_bytes(cd: np.ndarray, adj: np.ndarray, roots: np.ndarray) -> int:
     """Bytes one pass must move, per the vertex-rooted formulation: for each
-    root s with d+(s) >= k-1: its extent and key (16 B), and for each
-    out-neighbour x: the id (4 B), its extent and key (16 B) and its
-    adjacency slice (4 d+(x)); plus one 8-byte count write per node."""
+    scheduled root s: its extent and key (16 B), and for each out-neighbour
+    x: the id (4 B), its extent and key (16 B) and its adjacency slice
+    (4 d+(x)); plus one 8-byte count write per node."""
     dout = np.diff(cd).astype(np.int64)
     n = dout.size
-    roots = dout >= max(k - 1, 1)
     src = np.repeat(np.arange(n), dout)
     per_edge = 20 + 4 * dout[adj.astype(np.int64)]
-    b = 16 * int(roots.sum()) + int(per_edge[roots[src]].sum()) + 8 * n
-    return b, int(roots.sum())
+    return 16 * int(roots.sum()) + int(per_edge[roots[src]].sum()) + 8 * n
+
+
+def profiled_traffic(workload: str):
+    """HBM bytes per pass from the committed rocprofv3 summary of this
+    workload (FETCH_SIZE and WRITE_SIZE, scaled by the copy-kernel factors),
+    or None when there is none."""
+    path = os.path.join(ROOT, "profiles", f"pass_kernel_{workload}.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_pass:"):
+            return int(line.split(":")[1].split()[0])
+    return None
 
 
 def cpu_baseline(pairs: np.ndarray, k: int, target_s: float = 12.0):
@@ -114,6 +127,49 @@ def cpu_baseline_port(pairs, k, target_s):  # pragma: no cover - fallback only
             "sample": f"oracle restatement over {idx.size} random root edges, 1 thread, {dt:.1f}s"}
 
 
+def run_exact(args, kcl, gg):
+    """Time-to-exact: binary edge file -> graph -> certified maximal densest
+    subgraph (kcl_exact_run), wall clock."""
+    wname, seed, sd, k, desc = WORKLOADS[args.workload]
+    log(f"bench exact: generating {args.workload} ({desc})")
+    pairs = gg.workload(wname, seed=seed, scale_down=sd)
+    with tempfile.TemporaryDirectory() as td:
+        path = os.path.join(td, "edges.u32")
+        kcl.write_edge_list(path, pairs)
+        del pairs
+        t0 = time.perf_counter()
+        pr, n_in = kcl.read_edge_list(path)
+        t1 = time.perf_counter()
+        g = kcl.Graph(pr, n_in)
+        t2 = time.perf_counter()
+        log(f"bench exact: read {t1 - t0:.2f}s, graph n={g.n} m={g.m} core={g.core} "
+            f"{t2 - t1:.2f}s; running the exact pipeline")
+        res, nodes = kcl.exact(g, k, stat_path=os.path.join(td, "stat_exact.txt"),
+                               max_seconds=args.exact_budget)
+        t3 = time.perf_counter()
+        stat = open(os.path.join(td, "stat_exact.txt")).read()
+    log(stat)
+    out = {
+        "metric": f"time-to-exact k-CDS (k={k})",
+        "value": t3 - t0,
+        "unit": "s",
+        "n_gpus": 1,
+        "higher_is_better": False,
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic (seeded generator, kclistpp_amd/graphgen.py)",
+        "config": {"workload": f"{args.workload}: {desc}", "k": k, "n": g.n, "m": g.m,
+                   "core": g.core},
+        "phases_s": {"read": t1 - t0, "graph": t2 - t1, "exact": t3 - t2},
+        "result": res,
+        "cpu_baseline": None,
+        "cpu_baseline_note": "the reference's exact program (kCDensestMem.c) stores every "
+                             "clique and needs Boost.Graph, which is absent; not timed",
+    }
+    g.close()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -122,6 +178,10 @@ def main():
     ap.add_argument("--workload", default="cl1m", choices=sorted(WORKLOADS))
     ap.add_argument("--schedule", default="degree", choices=["degree", "learned"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--exact-budget", type=int, default=240,
+                    help="exact mode: seconds before giving up on a certificate")
+    ap.add_argument("--mode", default="sim", choices=["sim", "exact"],
+                    help="sim: visits/s of one iteration (default); exact: time-to-exact")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -131,6 +191,9 @@ def main():
     import kclistpp_amd as kcl
     from kclistpp_amd import graphgen as gg
 
+    if args.mode == "exact":
+        run_exact(args, kcl, gg)
+        return
     wname, seed, sd, k, desc = WORKLOADS[args.workload]
     log(f"bench: generating {args.workload} ({desc})")
     t0 = time.time()
@@ -140,11 +203,11 @@ def main():
     g = kcl.Graph(pairs)
     log(f"bench: graph n={g.n} m={g.m} core={g.core} built in {time.time() - t0:.1f}s")
     cd, adj = g.csr()
-    alg_bytes, nroots = algorithmic_bytes(cd, adj, k)
     sched = kcl.SCHED_LEARNED if args.schedule == "learned" else kcl.SCHED_DEGREE
     t0 = time.time()
     sim = kcl.Sim(g, k, schedule=sched)
     plan = sim.plan_info()
+    alg_bytes = algorithmic_bytes(cd, adj, sim.root_cliques() > 0)
     log(f"bench: plan ready in {time.time() - t0:.1f}s: {plan}; warmup {args.warmup}")
     t = 0
     for _ in range(args.warmup):
@@ -169,7 +232,8 @@ def main():
     kern_avg_ms = kern_ms / max(args.steps, 1)
     achieved = alg_bytes / (kern_avg_ms * 1e-3) / 1e9 if kern_avg_ms > 0 else 0.0
     roofline = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6),
+                "traffic": profiled_traffic(args.workload),
                 "kernel": "pass_kernel (all word classes of one enumeration pass)",
                 "kernel_ms": round(kern_avg_ms, 4), "algorithmic_bytes": alg_bytes}
     cpu = None
@@ -192,7 +256,7 @@ def main():
         "dtype": "u64",
         "data": "synthetic (seeded generator, kclistpp_amd/graphgen.py)",
         "config": {"workload": f"{args.workload}: {desc}", "k": k, "n": g.n, "m": g.m,
-                   "core": g.core, "roots": nroots, "visits_per_step": visits // max(args.steps, 1),
+                   "core": g.core, "roots": plan["roots_kept"], "visits_per_step": visits // max(args.steps, 1),
                    "schedule": args.schedule, "parallelism": "1 GPU",
                    "plan": plan},
         "roofline": roofline,

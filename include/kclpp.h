@@ -159,6 +159,9 @@ typedef struct kcl_plan_info {
 } kcl_plan_info;
 
 kcl_status kcl_sim_plan_info(kcl_sim *s, kcl_plan_info *info);
+/* k-cliques per root (n entries): the cliques whose largest id is that
+ * node; zero for nodes the plan does not schedule. */
+kcl_status kcl_sim_root_cliques(kcl_sim *s, uint64_t *out);
 
 /* One Frank-Wolfe pass at the current rho with per-task timing, leaving the
  * state unchanged.  *ntasks: in = room in the arrays, out = tasks of the
@@ -257,14 +260,18 @@ kcl_status kcl_trim_write(const kcl_trim *t, const char *ckdeg, const char *ckco
  * network, kCDensestMem.c:434-467) that either certifies D - returning the
  * maximal densest subgraph - or yields a denser candidate.  Rounds stop at
  * max_iter iterations; regions whose clique count exceeds
- * max_network_cliques are not sent to the max-flow.
+ * max_network_cliques are not sent to the max-flow.  Without a certificate
+ * the result is the densest subgraph found, with upper_bound = the largest
+ * average of the top-i normalised rho values (rho_T / (T(T+3)/((T+1)(T+2)))
+ * is a convex combination of clique-to-member allocations, so every set S
+ * has cliques(S) <= sum over S of it).
  * stat_path (nullable): the reference's stat_exact file (rows per round,
  * then the final record).  nodes_out (nullable, room for n entries of the
  * graph): input ids of the subgraph, ascending.
  * ------------------------------------------------------------------- */
 typedef struct kcl_exact_opts {
   uint32_t max_iter;             /* 0 = 65536 */
-  uint32_t reserved0;
+  uint32_t max_seconds;          /* wall-clock budget, 0 = none */
   uint64_t max_network_cliques;  /* 0 = 20,000,000 */
   uint32_t reserved[6];
 } kcl_exact_opts;
@@ -279,6 +286,8 @@ typedef struct kcl_exact_result {
   uint32_t max_flow_calls;
   uint32_t region;         /* nodes of the last region */
   double seconds;
+  double upper_bound;      /* bound on the maximum density from the last
+                              allocation (equals density when certified) */
 } kcl_exact_result;
 
 kcl_status kcl_exact_run(kcl_graph *g, uint32_t k, const kcl_exact_opts *opts,

@@ -73,14 +73,15 @@ class _TrimInfo(C.Structure):
 
 
 class _ExactOpts(C.Structure):
-    _fields_ = [("max_iter", C.c_uint32), ("reserved0", C.c_uint32),
+    _fields_ = [("max_iter", C.c_uint32), ("max_seconds", C.c_uint32),
                 ("max_network_cliques", C.c_uint64), ("reserved", C.c_uint32 * 6)]
 
 
 class _ExactResult(C.Structure):
     _fields_ = [("certified", C.c_uint32), ("num_iter", C.c_uint32), ("n", C.c_uint32),
                 ("m", C.c_uint32), ("cliques", C.c_uint64), ("density", C.c_double),
-                ("max_flow_calls", C.c_uint32), ("region", C.c_uint32), ("seconds", C.c_double)]
+                ("max_flow_calls", C.c_uint32), ("region", C.c_uint32), ("seconds", C.c_double),
+                ("upper_bound", C.c_double)]
 
 
 class _Record(C.Structure):
@@ -101,7 +102,7 @@ EXPORTS = [
     "kcl_graph_create", "kcl_graph_destroy", "kcl_graph_get_info", "kcl_graph_rank",
     "kcl_graph_csr", "kcl_graph_edges", "kcl_count_edges", "kcl_clique_enum", "kcl_sim_create",
     "kcl_sim_destroy", "kcl_sim_step", "kcl_sim_counts", "kcl_sim_rho", "kcl_sim_set_rho",
-    "kcl_sim_totals", "kcl_sim_last_pass", "kcl_sim_plan_info", "kcl_sim_profile_pass", "kcl_sim_find_densest", "kcl_cmpsync_run",
+    "kcl_sim_totals", "kcl_sim_last_pass", "kcl_sim_plan_info", "kcl_sim_root_cliques", "kcl_sim_profile_pass", "kcl_sim_find_densest", "kcl_cmpsync_run",
     "kcl_device_get_info", "kcl_version", "kcl_trim_create", "kcl_trim_destroy",
     "kcl_trim_get_info", "kcl_trim_nodes", "kcl_trim_edges", "kcl_trim_write", "kcl_exact_run",
 ]
@@ -142,6 +143,7 @@ def lib() -> C.CDLL:
         "kcl_sim_last_pass": ([vp, C.POINTER(u64), C.POINTER(C.c_double),
                                C.POINTER(C.c_double)], C.c_int),
         "kcl_sim_plan_info": ([vp, C.POINTER(_PlanInfo)], C.c_int),
+        "kcl_sim_root_cliques": ([vp, vp], C.c_int),
         "kcl_sim_profile_pass": ([vp, C.POINTER(u32), vp, vp, vp, vp, vp], C.c_int),
         "kcl_sim_find_densest": ([vp, C.POINTER(_Densest), vp, vp, vp], C.c_int),
         "kcl_cmpsync_run": ([vp, u32, u32, C.POINTER(_SimOpts), C.c_char_p, i64,
@@ -368,6 +370,11 @@ class Sim:
         return {"roots_total": p.roots_total, "roots_kept": p.roots_kept, "tasks": p.tasks,
                 "cliques": int(p.cliques), "plan_ms": p.plan_ms}
 
+    def root_cliques(self) -> np.ndarray:
+        out = np.zeros(max(self.graph.n, 1), np.uint64)
+        _check(lib().kcl_sim_root_cliques(self._h, _ptr(out)))
+        return out[:self.graph.n]
+
     def profile_pass(self) -> dict:
         """One pass with per-task shader-clock timing (state unchanged)."""
         nt = C.c_uint32(0)
@@ -466,12 +473,13 @@ class Trim:
 
 
 def exact(graph: Graph, k: int, max_iter: int = 0, max_network_cliques: int = 0,
-          stat_path: str | None = None, t0_unix: int = 0):
+          stat_path: str | None = None, t0_unix: int = 0, max_seconds: int = 0):
     """Exact k-clique densest subgraph (kCDensestMem's result).  Returns
     (result dict, input ids of the maximal densest subgraph)."""
     o = _ExactOpts()
     o.max_iter = max_iter
     o.max_network_cliques = max_network_cliques
+    o.max_seconds = max_seconds
     r = _ExactResult()
     nodes = np.zeros(max(graph.n, 1), np.uint32)
     _check(lib().kcl_exact_run(graph.handle, k, C.byref(o),

@@ -441,6 +441,7 @@ extern "C" kcl_status kcl_exact_run(kcl_graph *g, uint32_t k, const kcl_exact_op
   if (opts) {
     if (opts->max_iter) o.max_iter = opts->max_iter;
     if (opts->max_network_cliques) o.max_network_cliques = opts->max_network_cliques;
+    o.max_seconds = opts->max_seconds;
   }
   memset(res, 0, sizeof(*res));
   const auto c0 = std::chrono::steady_clock::now();
@@ -466,6 +467,21 @@ extern "C" kcl_status kcl_exact_run(kcl_graph *g, uint32_t k, const kcl_exact_op
     std::vector<uint32_t> reo;
     std::vector<double> rho;
     if ((st = kcl::sim_level_pass(sim, true, tent, reo, rho)) != KCL_OK) break;
+    {
+      // Upper bound: V_T / (T(T+3)/2) is a convex combination of
+      // allocations; the best top-i average of it bounds every density.
+      std::vector<uint64_t> V(n);
+      if ((st = kcl_sim_totals(sim, V.data())) != KCL_OK) break;
+      std::sort(V.begin(), V.end(), std::greater<uint64_t>());
+      const double w = (double)T * (double)(T + 3) / 2.0;
+      kcl::u128 acc = 0;
+      double ub = 0;
+      for (uint32_t i = 0; i < n; ++i) {
+        acc += V[i];
+        ub = std::max(ub, (double)acc / w / (double)(i + 1));
+      }
+      res->upper_bound = ub;
+    }
     double val0 = 0;
     const uint32_t q = kcl::pava_top_block(tent, &val0);
     uint64_t p = 0;
@@ -499,6 +515,10 @@ extern "C" kcl_status kcl_exact_run(kcl_graph *g, uint32_t k, const kcl_exact_op
               (long)(now - t0_unix), v.fits ? "STABLE BUT NOT DENSEST" : "NOT STABLE");
       fflush(f);
     }
+    if (o.max_seconds &&
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - c0).count() >
+            (double)o.max_seconds)
+      break;
   }
   if (st == KCL_OK && v.certified && !v.best.empty() && best_p != 0) {
     best = v.best;
@@ -511,6 +531,7 @@ extern "C" kcl_status kcl_exact_run(kcl_graph *g, uint32_t k, const kcl_exact_op
     return st;
   }
   res->certified = v.certified ? 1u : 0u;
+  if (v.certified) res->upper_bound = best_q ? (double)best_p / (double)best_q : 0.0;
   res->num_iter = iters;
   res->n = (uint32_t)best_q;
   res->cliques = best_p;

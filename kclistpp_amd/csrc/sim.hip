@@ -48,6 +48,7 @@ struct kcl_sim {
   uint64_t last_visits = 0;
   double last_kernel_ms = 0, last_pass_ms = 0;
   kcl_plan_info info{};
+  std::vector<uint64_t> root_cliques;  // k-cliques per root (plan filter)
 };
 
 namespace kcl {
@@ -270,7 +271,7 @@ kcl_status sim_prepare(kcl_sim *s) {
   if (!s->trivial) {
     const uint32_t split = s->opts.split_work ? s->opts.split_work : (1u << 17);
     PassPlan p;
-    KCL_TRY(make_plan(g, s->k, split, s->stream, &p, nullptr, &s->info));
+    KCL_TRY(make_plan(g, s->k, split, s->stream, &p, &s->root_cliques, &s->info));
     KCL_TRY(upload_plan(p, &s->plan));
   }
   KCL_HIP(hipStreamSynchronize(s->stream));
@@ -550,6 +551,13 @@ extern "C" kcl_status kcl_sim_profile_pass(kcl_sim *s, uint32_t *ntasks, uint32_
     if (build_cycles) build_cycles[i] = h[2 * (size_t)i];
     if (total_cycles) total_cycles[i] = h[2 * (size_t)i + 1];
   }
+  return KCL_OK;
+}
+
+extern "C" kcl_status kcl_sim_root_cliques(kcl_sim *s, uint64_t *out) {
+  if (!s || (!out && s->g->n)) return fail(KCL_E_INVALID_ARG, "NULL argument");
+  const uint32_t n = s->g->n;
+  for (uint32_t v = 0; v < n; ++v) out[v] = v < s->root_cliques.size() ? s->root_cliques[v] : 0;
   return KCL_OK;
 }
 
