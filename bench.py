@@ -144,8 +144,12 @@ def run_exact(args, kcl, gg):
         t2 = time.perf_counter()
         log(f"bench exact: read {t1 - t0:.2f}s, graph n={g.n} m={g.m} core={g.core} "
             f"{t2 - t1:.2f}s; running the exact pipeline")
+        model = load_model(args, kcl)
         res, nodes = kcl.exact(g, k, stat_path=os.path.join(td, "stat_exact.txt"),
-                               max_seconds=args.exact_budget)
+                               max_seconds=args.exact_budget,
+                               schedule=kcl.SCHEDULES[args.schedule], model=model)
+        if model is not None:
+            model.close()
         t3 = time.perf_counter()
         stat = open(os.path.join(td, "stat_exact.txt")).read()
     log(stat)
@@ -159,6 +163,7 @@ def run_exact(args, kcl, gg):
         "dtype": "u64",
         "data": "synthetic (seeded generator, kclistpp_amd/graphgen.py)",
         "config": {"workload": f"{args.workload}: {desc}", "k": k, "n": g.n, "m": g.m,
+                   "schedule": args.schedule,
                    "core": g.core},
         "phases_s": {"read": t1 - t0, "graph": t2 - t1, "exact": t3 - t2},
         "result": res,
@@ -176,7 +181,11 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="cl1m", choices=sorted(WORKLOADS))
-    ap.add_argument("--schedule", default="degree", choices=["degree", "learned"])
+    ap.add_argument("--schedule", default="counted", choices=["counted", "learned", "degree"],
+                    help="root order/splitting: counting-pass costs, the learned cost "
+                         "model, or plain d+^2 (counts are identical under all three)")
+    ap.add_argument("--model", default=None,
+                    help="learned schedule: cost model file (default: the shipped one)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--exact-budget", type=int, default=240,
                     help="exact mode: seconds before giving up on a certificate")
@@ -203,9 +212,10 @@ def main():
     g = kcl.Graph(pairs)
     log(f"bench: graph n={g.n} m={g.m} core={g.core} built in {time.time() - t0:.1f}s")
     cd, adj = g.csr()
-    sched = kcl.SCHED_LEARNED if args.schedule == "learned" else kcl.SCHED_DEGREE
+    sched = kcl.SCHEDULES[args.schedule]
+    model = load_model(args, kcl)
     t0 = time.time()
-    sim = kcl.Sim(g, k, schedule=sched)
+    sim = kcl.Sim(g, k, schedule=sched, model=model)
     plan = sim.plan_info()
     alg_bytes = algorithmic_bytes(cd, adj, sim.root_cliques() > 0)
     log(f"bench: plan ready in {time.time() - t0:.1f}s: {plan}; warmup {args.warmup}")
@@ -241,6 +251,8 @@ def main():
         log("bench: timing the reference on the host cores")
         cpu = cpu_baseline(pairs, k)
     sim.close()
+    if model is not None:
+        model.close()
     g.close()
     out = {
         "metric": "k-clique visits/sec per kClist++ iteration",

@@ -120,16 +120,25 @@ kcl_status kcl_clique_enum(kcl_graph *g, uint32_t k, kcl_task task, const double
  * ------------------------------------------------------------------- */
 typedef struct kcl_sim kcl_sim;
 
+/* How a pass orders and splits its roots.  The schedule changes only the
+ * time of a pass, never its counts.  The reference's counterpart is the
+ * OpenMP dynamic schedule over edges (kCDensestCmpSync.c:300-305). */
 typedef enum kcl_schedule {
-  KCL_SCHED_DEGREE = 0, /* roots by estimated cost from degrees, heavy first */
-  KCL_SCHED_LEARNED = 1 /* roots ordered and split by the cost model (falls
-                           back to KCL_SCHED_DEGREE when none is loaded) */
+  KCL_SCHED_COUNTED = 0, /* cost from a root-credit (k-1)-clique counting
+                            pass (steps x row words + build entries) */
+  KCL_SCHED_LEARNED = 1, /* cost predicted by a kcl_cost_model from eight
+                            per-root features (falls back to
+                            KCL_SCHED_DEGREE when no model is given) */
+  KCL_SCHED_DEGREE = 2   /* plain: cost d+(s)^2, heavy first */
 } kcl_schedule;
+
+typedef struct kcl_cost_model kcl_cost_model;
 
 typedef struct kcl_sim_opts {
   uint32_t schedule;      /* kcl_schedule */
-  uint32_t split_work;    /* target work units per split root; 0 = default */
-  uint32_t reserved[6];
+  uint32_t split_work;    /* COUNTED/DEGREE: work units per split part; 0 = default */
+  const kcl_cost_model *model; /* LEARNED: the model (nullable) */
+  uint32_t reserved[4];
 } kcl_sim_opts;
 
 kcl_status kcl_sim_create(kcl_graph *g, uint32_t k, const kcl_sim_opts *opts, kcl_sim **out);
@@ -170,6 +179,71 @@ kcl_status kcl_sim_root_cliques(kcl_sim *s, uint64_t *out);
  * rows were built, cycles of the whole task. */
 kcl_status kcl_sim_profile_pass(kcl_sim *s, uint32_t *ntasks, uint32_t *root, uint32_t *part,
                                 uint32_t *nparts, uint64_t *build_cycles, uint64_t *total_cycles);
+
+/* ---------------------------------------------------------------------
+ * Learned root-cost model (no counterpart in the reference, whose OpenMP
+ * loop has no cost estimate).  A BF16 MLP 8 -> 64 -> 64 -> 2 (ReLU) run
+ * and trained on the GPU by the library's own MFMA kernels (fp32 master
+ * weights, fp32 accumulation, Adam).  Inputs are eight features of a root
+ * s (kcl_sim_root_features); outputs are log2(1 + cycles) of building the
+ * root's rows and of counting its cliques, which the LEARNED schedule
+ * turns into task order and split counts.
+ *
+ * Parameter vector (KCL_CM_PARAMS floats): W1[8][64], b1[64], W2[64][64],
+ * b2[64], W3[64][2], b3[2] (row = input unit).  Normalisation
+ * (KCL_CM_NORM floats): feature mean[8], std[8], target mean[2], std[2];
+ * the network sees (x - mean)/std and predicts (y - mean)/std.
+ * ------------------------------------------------------------------- */
+#define KCL_CM_FEATURES 8
+#define KCL_CM_HIDDEN 64
+#define KCL_CM_OUTPUTS 2
+#define KCL_CM_PARAMS 4866
+#define KCL_CM_NORM 20
+
+/* Fresh model: uniform(-1/sqrt(fan_in), 1/sqrt(fan_in)) weights from a
+ * splitmix64 stream of `seed`, zero biases, identity normalisation. */
+kcl_status kcl_cost_model_create(uint64_t seed, kcl_cost_model **out);
+void kcl_cost_model_destroy(kcl_cost_model *m);
+/* Binary file: "KCLCM001", then KCL_CM_NORM and KCL_CM_PARAMS
+ * little-endian floats. */
+kcl_status kcl_cost_model_load(const char *path, kcl_cost_model **out);
+kcl_status kcl_cost_model_save(const kcl_cost_model *m, const char *path);
+kcl_status kcl_cost_model_get(const kcl_cost_model *m, float *params, float *norm);
+kcl_status kcl_cost_model_set(kcl_cost_model *m, const float *params, const float *norm);
+/* y[rows][2] = de-normalised predictions for x[rows][8] (GPU). */
+kcl_status kcl_cost_model_predict(kcl_cost_model *m, const float *x, uint64_t rows, float *y);
+/* Loss L = sum over rows and outputs of (out - y_norm)^2 / (2 rows) and its
+ * gradient with respect to the parameter vector (GPU forward + backward). */
+kcl_status kcl_cost_model_gradient(kcl_cost_model *m, const float *x, const float *y,
+                                   uint64_t rows, float *loss, float *grad);
+
+typedef struct kcl_cm_train_opts {
+  uint32_t epochs;   /* 0 = 60 */
+  uint32_t batch;    /* rows per step, 0 = 2048 */
+  float lr;          /* Adam step, 0 = 2e-3 */
+  uint32_t fit_norm; /* 1: set the normalisation from x and y first */
+  uint64_t seed;     /* row shuffle */
+  uint32_t reserved[4];
+} kcl_cm_train_opts;
+
+/* Adam on minibatches of shuffled rows (GPU).  loss_per_epoch (nullable,
+ * room for epochs floats) receives each epoch's mean loss. */
+kcl_status kcl_cost_model_train(kcl_cost_model *m, const float *x, const float *y, uint64_t rows,
+                                const kcl_cm_train_opts *opts, float *loss_per_epoch);
+
+/* Features of every node as a root of this sim's k (n rows of 8 floats):
+ * log2(1+d+), log2(1+core), log2(1 + sum of the out-neighbours' d+),
+ * log2(1 + edges inside N+(s)), edges inside N+(s) / C(d+, 2),
+ * log2(1 + largest out-neighbour d+), log2(row words), k. */
+kcl_status kcl_sim_root_features(kcl_sim *s, float *x);
+/* Training rows from one timed pass (kcl_sim_profile_pass) of the current
+ * plan: one row per scheduled root, with its features and labels
+ * log2(1 + build cycles) (mean over the root's parts) and
+ * log2(1 + counting cycles) (summed over its parts).  *nrows: in = room,
+ * out = rows (all arrays NULL: size query). */
+kcl_status kcl_sim_cost_samples(kcl_sim *s, uint32_t *nrows, uint32_t *root, float *x, float *y);
+/* Rebuild the plan with another schedule (model: for KCL_SCHED_LEARNED). */
+kcl_status kcl_sim_replan(kcl_sim *s, uint32_t schedule, const kcl_cost_model *model);
 
 typedef struct kcl_densest_info {
   uint32_t n;       /* nodes of the densest prefix */
@@ -273,7 +347,10 @@ typedef struct kcl_exact_opts {
   uint32_t max_iter;             /* 0 = 65536 */
   uint32_t max_seconds;          /* wall-clock budget, 0 = none */
   uint64_t max_network_cliques;  /* 0 = 20,000,000 */
-  uint32_t reserved[6];
+  uint32_t schedule;             /* kcl_schedule of the Sim iterations */
+  uint32_t reserved0;
+  const kcl_cost_model *model;   /* for KCL_SCHED_LEARNED (nullable) */
+  uint32_t reserved[2];
 } kcl_exact_opts;
 
 typedef struct kcl_exact_result {

@@ -26,8 +26,14 @@ LIB_PATH = os.path.join(PKG, "libkclpp.so")
 
 FRANK_WOLFE = 2
 PAVA_PREPROCESS = 3
-SCHED_DEGREE = 0
+SCHED_COUNTED = 0
 SCHED_LEARNED = 1
+SCHED_DEGREE = 2
+SCHEDULES = {"counted": SCHED_COUNTED, "learned": SCHED_LEARNED, "degree": SCHED_DEGREE}
+CM_FEATURES = 8
+CM_OUTPUTS = 2
+CM_PARAMS = 4866
+CM_NORM = 20
 
 _STATUS = {
     0: "KCL_OK", 1: "KCL_E_IO", 2: "KCL_E_PARSE", 3: "KCL_E_INVALID_ARG",
@@ -49,8 +55,13 @@ class _GraphInfo(C.Structure):
 
 
 class _SimOpts(C.Structure):
-    _fields_ = [("schedule", C.c_uint32), ("split_work", C.c_uint32),
-                ("reserved", C.c_uint32 * 6)]
+    _fields_ = [("schedule", C.c_uint32), ("split_work", C.c_uint32), ("model", C.c_void_p),
+                ("reserved", C.c_uint32 * 4)]
+
+
+class _TrainOpts(C.Structure):
+    _fields_ = [("epochs", C.c_uint32), ("batch", C.c_uint32), ("lr", C.c_float),
+                ("fit_norm", C.c_uint32), ("seed", C.c_uint64), ("reserved", C.c_uint32 * 4)]
 
 
 class _Densest(C.Structure):
@@ -74,7 +85,8 @@ class _TrimInfo(C.Structure):
 
 class _ExactOpts(C.Structure):
     _fields_ = [("max_iter", C.c_uint32), ("max_seconds", C.c_uint32),
-                ("max_network_cliques", C.c_uint64), ("reserved", C.c_uint32 * 6)]
+                ("max_network_cliques", C.c_uint64), ("schedule", C.c_uint32),
+                ("reserved0", C.c_uint32), ("model", C.c_void_p), ("reserved", C.c_uint32 * 2)]
 
 
 class _ExactResult(C.Structure):
@@ -105,6 +117,10 @@ EXPORTS = [
     "kcl_sim_totals", "kcl_sim_last_pass", "kcl_sim_plan_info", "kcl_sim_root_cliques", "kcl_sim_profile_pass", "kcl_sim_find_densest", "kcl_cmpsync_run",
     "kcl_device_get_info", "kcl_version", "kcl_trim_create", "kcl_trim_destroy",
     "kcl_trim_get_info", "kcl_trim_nodes", "kcl_trim_edges", "kcl_trim_write", "kcl_exact_run",
+    "kcl_cost_model_create", "kcl_cost_model_destroy", "kcl_cost_model_load",
+    "kcl_cost_model_save", "kcl_cost_model_get", "kcl_cost_model_set", "kcl_cost_model_predict",
+    "kcl_cost_model_gradient", "kcl_cost_model_train", "kcl_sim_root_features",
+    "kcl_sim_cost_samples", "kcl_sim_replan",
 ]
 
 
@@ -158,6 +174,18 @@ def lib() -> C.CDLL:
                             i64], C.c_int),
         "kcl_exact_run": ([vp, u32, C.POINTER(_ExactOpts), C.c_char_p, i64,
                            C.POINTER(_ExactResult), vp], C.c_int),
+        "kcl_cost_model_create": ([u64, C.POINTER(vp)], C.c_int),
+        "kcl_cost_model_destroy": ([vp], None),
+        "kcl_cost_model_load": ([C.c_char_p, C.POINTER(vp)], C.c_int),
+        "kcl_cost_model_save": ([vp, C.c_char_p], C.c_int),
+        "kcl_cost_model_get": ([vp, vp, vp], C.c_int),
+        "kcl_cost_model_set": ([vp, vp, vp], C.c_int),
+        "kcl_cost_model_predict": ([vp, vp, u64, vp], C.c_int),
+        "kcl_cost_model_gradient": ([vp, vp, vp, u64, C.POINTER(C.c_float), vp], C.c_int),
+        "kcl_cost_model_train": ([vp, vp, vp, u64, C.POINTER(_TrainOpts), vp], C.c_int),
+        "kcl_sim_root_features": ([vp, vp], C.c_int),
+        "kcl_sim_cost_samples": ([vp, C.POINTER(u32), vp, vp, vp], C.c_int),
+        "kcl_sim_replan": ([vp, u32, vp], C.c_int),
     }
     for name, (args, res) in sigs.items():
         f = getattr(L, name)
@@ -296,21 +324,115 @@ class Graph:
         return acc[:self.n], int(v.value)
 
 
-def _opts(schedule: int = SCHED_DEGREE, split_work: int = 0):
+def _opts(schedule: int = SCHED_COUNTED, split_work: int = 0, model: "CostModel | None" = None):
     o = _SimOpts()
     o.schedule = schedule
     o.split_work = split_work
+    o.model = model.handle if model is not None else None
     return o
+
+
+def _f32(a, cols: int) -> np.ndarray:
+    a = np.ascontiguousarray(a, np.float32)
+    if a.ndim != 2 or a.shape[1] != cols:
+        raise ValueError(f"expected an (rows, {cols}) array, got shape {a.shape}")
+    return a
+
+
+class CostModel:
+    """Learned root-cost model: BF16 MLP 8 -> 64 -> 64 -> 2 with GPU training
+    and inference kernels (kcl_cost_model_*)."""
+
+    def __init__(self, seed: int = 0, path: str | None = None):
+        h = C.c_void_p()
+        if path is not None:
+            _check(lib().kcl_cost_model_load(path.encode(), C.byref(h)))
+        else:
+            _check(lib().kcl_cost_model_create(seed, C.byref(h)))
+        self._h = h
+
+    @property
+    def handle(self):
+        if not self._h:
+            raise KclError(3, "cost model is closed")
+        return self._h
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().kcl_cost_model_destroy(self._h)
+            self._h = None
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def save(self, path: str) -> None:
+        _check(lib().kcl_cost_model_save(self.handle, path.encode()))
+
+    def get(self) -> tuple[np.ndarray, np.ndarray]:
+        p = np.zeros(CM_PARAMS, np.float32)
+        nrm = np.zeros(CM_NORM, np.float32)
+        _check(lib().kcl_cost_model_get(self.handle, _ptr(p), _ptr(nrm)))
+        return p, nrm
+
+    def set(self, params: np.ndarray | None = None, norm: np.ndarray | None = None) -> None:
+        p = None if params is None else np.ascontiguousarray(params, np.float32)
+        nrm = None if norm is None else np.ascontiguousarray(norm, np.float32)
+        if p is not None and p.size != CM_PARAMS:
+            raise ValueError(f"params: expected {CM_PARAMS} floats")
+        if nrm is not None and nrm.size != CM_NORM:
+            raise ValueError(f"norm: expected {CM_NORM} floats")
+        _check(lib().kcl_cost_model_set(self.handle, _ptr(p), _ptr(nrm)))
+
+    def predict(self, x: np.ndarray) -> np.ndarray:
+        x = _f32(x, CM_FEATURES)
+        y = np.zeros((max(len(x), 1), CM_OUTPUTS), np.float32)
+        _check(lib().kcl_cost_model_predict(self.handle, _ptr(x), len(x), _ptr(y)))
+        return y[:len(x)]
+
+    def gradient(self, x: np.ndarray, y: np.ndarray) -> tuple[float, np.ndarray]:
+        x = _f32(x, CM_FEATURES)
+        y = _f32(y, CM_OUTPUTS)
+        if len(x) != len(y):
+            raise ValueError("x and y differ in rows")
+        g = np.zeros(CM_PARAMS, np.float32)
+        loss = C.c_float(0)
+        _check(lib().kcl_cost_model_gradient(self.handle, _ptr(x), _ptr(y), len(x),
+                                             C.byref(loss), _ptr(g)))
+        return float(loss.value), g
+
+    def train(self, x: np.ndarray, y: np.ndarray, epochs: int = 0, batch: int = 0,
+              lr: float = 0.0, fit_norm: bool = True, seed: int = 0) -> np.ndarray:
+        """Adam on the GPU; returns the mean loss of each epoch."""
+        x = _f32(x, CM_FEATURES)
+        y = _f32(y, CM_OUTPUTS)
+        if len(x) != len(y):
+            raise ValueError("x and y differ in rows")
+        o = _TrainOpts()
+        o.epochs, o.batch, o.lr, o.fit_norm, o.seed = epochs, batch, lr, int(fit_norm), seed
+        losses = np.zeros(epochs or 60, np.float32)
+        _check(lib().kcl_cost_model_train(self.handle, _ptr(x), _ptr(y), len(x), C.byref(o),
+                                          _ptr(losses)))
+        return losses
 
 
 class Sim:
     """Device-resident Sim-kClist++ state for one graph and k."""
 
-    def __init__(self, graph: Graph, k: int, schedule: int = SCHED_DEGREE, split_work: int = 0):
+    def __init__(self, graph: Graph, k: int, schedule: int = SCHED_COUNTED, split_work: int = 0,
+                 model: CostModel | None = None):
         self.graph = graph
         self.k = k
         h = C.c_void_p()
-        o = _opts(schedule, split_work)
+        o = _opts(schedule, split_work, model)
         _check(lib().kcl_sim_create(graph.handle, k, C.byref(o), C.byref(h)))
         self._h = h
         self.t = 0
@@ -389,6 +511,27 @@ class Sim:
                                           _ptr(out["build_cycles"]), _ptr(out["total_cycles"])))
         return {k: v[:n] for k, v in out.items()}
 
+    def root_features(self) -> np.ndarray:
+        """Cost-model features of every node as a root ((n, 8) float32)."""
+        x = np.zeros((max(self.graph.n, 1), CM_FEATURES), np.float32)
+        _check(lib().kcl_sim_root_features(self._h, _ptr(x)))
+        return x[:self.graph.n]
+
+    def cost_samples(self) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+        """(roots, features, labels) from one timed pass of the current plan."""
+        nr = C.c_uint32(0)
+        _check(lib().kcl_sim_cost_samples(self._h, C.byref(nr), None, None, None))
+        n = nr.value
+        root = np.zeros(max(n, 1), np.uint32)
+        x = np.zeros((max(n, 1), CM_FEATURES), np.float32)
+        y = np.zeros((max(n, 1), CM_OUTPUTS), np.float32)
+        _check(lib().kcl_sim_cost_samples(self._h, C.byref(nr), _ptr(root), _ptr(x), _ptr(y)))
+        return root[:n], x[:n], y[:n]
+
+    def replan(self, schedule: int, model: CostModel | None = None) -> None:
+        _check(lib().kcl_sim_replan(self._h, schedule,
+                                    model.handle if model is not None else None))
+
     def find_densest(self, want_vectors: bool = False):
         d = _Densest()
         n = max(self.graph.n, 1)
@@ -404,11 +547,12 @@ class Sim:
 
 
 def cmpsync(graph: Graph, k: int, num_iter: int, stat_path: str | None = None,
-            schedule: int = SCHED_DEGREE, t0_unix: int = 0) -> list[dict]:
+            schedule: int = SCHED_COUNTED, t0_unix: int = 0,
+            model: CostModel | None = None) -> list[dict]:
     """CDF_Main: one record per power of two <= num_iter."""
     recs = (_Record * 32)()
     nrec = C.c_uint32(0)
-    o = _opts(schedule)
+    o = _opts(schedule, 0, model)
     _check(lib().kcl_cmpsync_run(graph.handle, k, num_iter, C.byref(o),
                                  stat_path.encode() if stat_path else None, t0_unix, recs,
                                  C.byref(nrec)))
@@ -473,13 +617,16 @@ class Trim:
 
 
 def exact(graph: Graph, k: int, max_iter: int = 0, max_network_cliques: int = 0,
-          stat_path: str | None = None, t0_unix: int = 0, max_seconds: int = 0):
+          stat_path: str | None = None, t0_unix: int = 0, max_seconds: int = 0,
+          schedule: int = SCHED_COUNTED, model: CostModel | None = None):
     """Exact k-clique densest subgraph (kCDensestMem's result).  Returns
     (result dict, input ids of the maximal densest subgraph)."""
     o = _ExactOpts()
     o.max_iter = max_iter
     o.max_network_cliques = max_network_cliques
     o.max_seconds = max_seconds
+    o.schedule = schedule
+    o.model = model.handle if model is not None else None
     r = _ExactResult()
     nodes = np.zeros(max(graph.n, 1), np.uint32)
     _check(lib().kcl_exact_run(graph.handle, k, C.byref(o),

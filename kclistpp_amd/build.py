@@ -35,7 +35,8 @@ HIP_FLAGS = [
     "-Wall", "-Wno-unused-function", "-I", INCLUDE, "-I", CSRC,
 ]
 
-SOURCES = ["host_io.cpp", "host_order.cpp", "graph.hip", "pass.hip", "sim.hip", "trim.cpp", "exact.cpp"]
+SOURCES = ["host_io.cpp", "host_order.cpp", "graph.hip", "pass.hip", "plan.hip", "sim.hip",
+           "costmodel.hip", "trim.cpp", "exact.cpp"]
 CLIS = {
     "kCDensestCmpSync": "cli_cmpsync.c",
     "kClistCoreTrim": "cli_trim.c",

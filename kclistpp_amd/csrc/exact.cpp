@@ -442,6 +442,8 @@ extern "C" kcl_status kcl_exact_run(kcl_graph *g, uint32_t k, const kcl_exact_op
     if (opts->max_iter) o.max_iter = opts->max_iter;
     if (opts->max_network_cliques) o.max_network_cliques = opts->max_network_cliques;
     o.max_seconds = opts->max_seconds;
+    o.schedule = opts->schedule;
+    o.model = opts->model;
   }
   memset(res, 0, sizeof(*res));
   const auto c0 = std::chrono::steady_clock::now();
@@ -454,7 +456,10 @@ extern "C" kcl_status kcl_exact_run(kcl_graph *g, uint32_t k, const kcl_exact_op
   }
   const uint32_t n = g->n;
   kcl_sim *sim = nullptr;
-  kcl_status st = kcl_sim_create(g, k, nullptr, &sim);
+  kcl_sim_opts so{};
+  so.schedule = o.schedule;
+  so.model = o.model;
+  kcl_status st = kcl_sim_create(g, k, &so, &sim);
   kcl::Verdict v;
   uint32_t flows = 0, region = 0, iters = 0;
   uint64_t best_p = 0, best_q = 0;

@@ -133,10 +133,28 @@ kcl_status build_plan(const std::vector<uint32_t> &cd, uint32_t k, const RootCos
                       const std::vector<uint64_t> *keep = nullptr);
 kcl_status upload_plan(const PassPlan &plan, DevicePlan *dev);
 
-// Plan for (graph, k): cost pass, root filter, split tasks.  root_cliques
-// (nullable) receives each root's k-clique count, info (nullable) a summary.
+// Concurrent task slots of a word class in one launch (workgroups).
+uint32_t class_slots(uint32_t wclass);
+
+// Plan for (graph, k): root filter (a root-credit k-pass), costs by the
+// schedule, split tasks.  root_cliques (nullable) receives each root's
+// k-clique count, info (nullable) a summary.  LEARNED without a model is
+// DEGREE.
 kcl_status make_plan(kcl_graph *g, uint32_t k, uint32_t split, hipStream_t st, PassPlan *out,
-                     std::vector<uint64_t> *root_cliques, kcl_plan_info *info);
+                     std::vector<uint64_t> *root_cliques, kcl_plan_info *info,
+                     uint32_t schedule = KCL_SCHED_COUNTED,
+                     const kcl_cost_model *model = nullptr);
+// Tasks for the roots with keep[s] != 0 under a schedule.
+kcl_status plan_for_schedule(kcl_graph *g, uint32_t k, uint32_t schedule,
+                             const kcl_cost_model *model, uint32_t split,
+                             const std::vector<uint64_t> *keep, hipStream_t st, PassPlan *out);
+// Cost-model features of every node as a root (n rows of KCL_CM_FEATURES).
+kcl_status root_features(kcl_graph *g, uint32_t k, hipStream_t st, std::vector<float> *out);
+// Cost-model predictions for device-resident rows (log2(1 + cycles)).
+kcl_status cost_model_predict_device(const kcl_cost_model *m, const float *d_x, uint64_t rows,
+                                     float *d_y, hipStream_t st);
+// Fills n u64 with zero on `st`.
+kcl_status zero_u64(unsigned long long *p, uint64_t n, hipStream_t st);
 
 kcl_status sim_step(kcl_sim *s, uint32_t t);
 kcl_status sim_level_pass(kcl_sim *s, bool full_key, std::vector<unsigned long long> &tent,

@@ -37,6 +37,24 @@ size_t lds_bytes(uint32_t wclass) { return pass_table().lds[wclass]; }
 
 uint32_t max_pass_k() { return (uint32_t)kMaxTemplK; }
 
+namespace {
+
+uint32_t blocks_per_cu(uint32_t wclass, const DeviceProps &props) {
+  const uint32_t waves = kClassWaves[wclass];
+  uint32_t per_cu = (uint32_t)(props.lds_per_cu / (lds_bytes(wclass) + 256));
+  if (per_cu < 1) per_cu = 1;
+  if (per_cu > 32 / waves) per_cu = 32 / waves;
+  return per_cu;
+}
+
+}  // namespace
+
+uint32_t class_slots(uint32_t wclass) {
+  DeviceProps props;
+  if (wclass >= kNumClasses || device_props(&props) != KCL_OK) return 1;
+  return props.cu_count * blocks_per_cu(wclass, props);
+}
+
 kcl_status launch_pass(const kcl_graph *g, const uint64_t *d_key, uint32_t k, PassMode mode,
                        DevicePlan &plan, unsigned long long *d_cnt, hipStream_t stream,
                        unsigned long long *prof) {
@@ -48,12 +66,8 @@ kcl_status launch_pass(const kcl_graph *g, const uint64_t *d_key, uint32_t k, Pa
     if (!count) continue;
     PassFn fn = kernel_for((uint32_t)c, k, mode == kCreditAll);
     if (!fn) return fail(KCL_E_UNSUPPORTED, "no enumeration kernel for k = %u", k);
-    const size_t lds = lds_bytes((uint32_t)c);
     const uint32_t waves = kClassWaves[c];
-    uint32_t per_cu = (uint32_t)(props.lds_per_cu / (lds + 256));
-    if (per_cu < 1) per_cu = 1;
-    if (per_cu > 32 / waves) per_cu = 32 / waves;
-    uint64_t blocks = (uint64_t)props.cu_count * per_cu;
+    uint64_t blocks = (uint64_t)props.cu_count * blocks_per_cu((uint32_t)c, props);
     if (blocks > count) blocks = count;
     hipLaunchKernelGGL(fn, dim3((uint32_t)blocks), dim3(64 * waves), 0, stream,
                        (const uint32_t *)g->d_cd.p, (const uint32_t *)g->d_adj.p, d_key,

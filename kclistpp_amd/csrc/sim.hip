@@ -160,93 +160,12 @@ __global__ void fill_u64_kernel(unsigned long long *p, uint64_t n, unsigned long
     p[i] = v;
 }
 
+}  // namespace
+
 kcl_status zero_u64(unsigned long long *p, uint64_t n, hipStream_t st) {
   if (!n) return KCL_OK;
   fill_u64_kernel<<<grid_for(n, 256), 256, 0, st>>>(p, n, 0ull);
   KCL_HIP(hipGetLastError());
-  return KCL_OK;
-}
-
-// Per-root clique counts: a root-credit pass of k-cliques over `plan`
-// (out[s] = number of k-cliques whose largest id is s).
-kcl_status root_counts(kcl_graph *g, uint32_t k, const PassPlan &plan, hipStream_t st,
-                       std::vector<uint64_t> *out) {
-  out->assign(g->n, 0);
-  DevicePlan dp;
-  KCL_TRY(upload_plan(plan, &dp));
-  DevBuf<unsigned long long> c;
-  DevBuf<unsigned long long> key;
-  KCL_TRY(c.alloc(g->n));
-  KCL_TRY(key.alloc(g->n));
-  KCL_TRY(zero_u64(c.p, g->n, st));
-  KCL_TRY(zero_u64(key.p, g->n, st));
-  KCL_TRY(launch_pass(g, (const uint64_t *)key.p, k, kCreditRoot, dp, c.p,
-                      st));
-  KCL_HIP(hipMemcpyAsync(out->data(), c.p, (size_t)g->n * sizeof(uint64_t),
-                         hipMemcpyDeviceToHost, st));
-  KCL_HIP(hipStreamSynchronize(st));
-  return KCL_OK;
-}
-
-// Work per root: the steps of its k-pass (one per (k-1)-clique with that
-// root, counted by a root-credit (k-1)-pass) times the row width, plus the
-// adjacency entries its build scans (so light roots keep a weight and split
-// parts amortise their rebuild).
-kcl_status root_costs(kcl_graph *g, uint32_t k, hipStream_t st, RootCost *out) {
-  PassPlan p;
-  KCL_TRY(build_plan(g->cd, k - 1, nullptr, 0, &p));
-  KCL_TRY(root_counts(g, k - 1, p, st, &out->work));
-  out->build.assign(g->n, 0);
-  for (uint32_t s = 0; s < g->n; ++s) {
-    uint64_t b = 0;
-    for (uint32_t j = g->cd[s]; j < g->cd[s + 1]; ++j) {
-      const uint32_t x = g->adj[j];
-      b += g->cd[x + 1] - g->cd[x];
-    }
-    out->build[s] = b;
-    out->work[s] *= (g->cd[s + 1] - g->cd[s] + 63) / 64;
-  }
-  return KCL_OK;
-}
-
-}  // namespace
-
-// The plan is fixed per graph and k (keys only change who wins):
-//  1. cost per root from a root-credit (k-1)-pass (k >= 4);
-//  2. a root-credit k-pass gives each root's k-clique count; roots with
-//     none are dropped (exact: they contribute nothing to any pass);
-//  3. tasks for the remaining roots, heavy ones split by cost.
-kcl_status make_plan(kcl_graph *g, uint32_t k, uint32_t split, hipStream_t st, PassPlan *out,
-                     std::vector<uint64_t> *root_cliques, kcl_plan_info *info) {
-  const auto t0 = std::chrono::steady_clock::now();
-  const uint32_t n = g->n;
-  RootCost cost;
-  const RootCost *cp = nullptr;
-  if (k >= 4) {
-    KCL_TRY(root_costs(g, k, st, &cost));
-    cp = &cost;
-  }
-  PassPlan all;
-  KCL_TRY(build_plan(g->cd, k, cp, split, &all));
-  std::vector<uint64_t> ck;
-  KCL_TRY(root_counts(g, k, all, st, &ck));
-  KCL_TRY(build_plan(g->cd, k, cp, split, out, &ck));
-  if (info) {
-    uint64_t cliques = 0;
-    uint32_t kept = 0, roots_all = 0;
-    for (uint32_t v = 0; v < n; ++v) {
-      cliques += ck[v];
-      kept += ck[v] ? 1u : 0u;
-      roots_all += (g->cd[v + 1] - g->cd[v] >= k - 1) ? 1u : 0u;
-    }
-    info->roots_total = roots_all;
-    info->roots_kept = kept;
-    info->tasks = (uint32_t)out->tasks.size();
-    info->cliques = cliques;
-    info->plan_ms =
-        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-  }
-  if (root_cliques) root_cliques->swap(ck);
   return KCL_OK;
 }
 
@@ -271,7 +190,8 @@ kcl_status sim_prepare(kcl_sim *s) {
   if (!s->trivial) {
     const uint32_t split = s->opts.split_work ? s->opts.split_work : (1u << 17);
     PassPlan p;
-    KCL_TRY(make_plan(g, s->k, split, s->stream, &p, &s->root_cliques, &s->info));
+    KCL_TRY(make_plan(g, s->k, split, s->stream, &p, &s->root_cliques, &s->info,
+                      s->opts.schedule, s->opts.model));
     KCL_TRY(upload_plan(p, &s->plan));
   }
   KCL_HIP(hipStreamSynchronize(s->stream));
@@ -558,6 +478,81 @@ extern "C" kcl_status kcl_sim_root_cliques(kcl_sim *s, uint64_t *out) {
   if (!s || (!out && s->g->n)) return fail(KCL_E_INVALID_ARG, "NULL argument");
   const uint32_t n = s->g->n;
   for (uint32_t v = 0; v < n; ++v) out[v] = v < s->root_cliques.size() ? s->root_cliques[v] : 0;
+  return KCL_OK;
+}
+
+extern "C" kcl_status kcl_sim_root_features(kcl_sim *s, float *x) {
+  if (!s || (!x && s->g->n)) return fail(KCL_E_INVALID_ARG, "NULL argument");
+  std::vector<float> f;
+  KCL_TRY(kcl::root_features(s->g, s->k, s->stream, &f));
+  if (!f.empty()) std::memcpy(x, f.data(), f.size() * sizeof(float));
+  return KCL_OK;
+}
+
+extern "C" kcl_status kcl_sim_cost_samples(kcl_sim *s, uint32_t *nrows, uint32_t *root, float *x,
+                                           float *y) {
+  if (!s || !nrows) return fail(KCL_E_INVALID_ARG, "NULL argument");
+  const std::vector<kcl::Task> &tasks = s->plan.host_tasks;
+  std::vector<uint8_t> seen(s->g->n, 0);
+  uint32_t roots = 0;
+  for (const kcl::Task &t : tasks)
+    if (!seen[t.s]) {
+      seen[t.s] = 1;
+      ++roots;
+    }
+  const uint32_t cap = *nrows;
+  *nrows = roots;
+  if (!root && !x && !y) return KCL_OK;  // size query
+  if (cap < roots) return fail(KCL_E_INVALID_ARG, "room for %u rows, the plan has %u roots", cap,
+                               roots);
+  if (roots == 0) return KCL_OK;
+  uint32_t nt = (uint32_t)tasks.size();
+  std::vector<uint32_t> troot(nt), tparts(nt);
+  std::vector<uint64_t> tb(nt), tt(nt);
+  KCL_TRY(kcl_sim_profile_pass(s, &nt, troot.data(), nullptr, tparts.data(), tb.data(),
+                               tt.data()));
+  std::vector<double> bsum(s->g->n, 0.0), csum(s->g->n, 0.0);
+  std::vector<uint32_t> parts(s->g->n, 0);
+  for (uint32_t i = 0; i < nt; ++i) {
+    const uint32_t v = troot[i];
+    bsum[v] += (double)tb[i];
+    csum[v] += (double)(tt[i] > tb[i] ? tt[i] - tb[i] : 0);
+    parts[v] += 1;
+  }
+  std::vector<float> feat;
+  if (x) KCL_TRY(kcl::root_features(s->g, s->k, s->stream, &feat));
+  uint32_t r = 0;
+  for (uint32_t v = 0; v < s->g->n; ++v) {  // rows by ascending root id
+    if (!parts[v]) continue;
+    if (root) root[r] = v;
+    if (x)
+      std::memcpy(x + (size_t)r * KCL_CM_FEATURES, feat.data() + (size_t)v * KCL_CM_FEATURES,
+                  KCL_CM_FEATURES * sizeof(float));
+    if (y) {
+      y[2 * (size_t)r] = (float)std::log2(1.0 + bsum[v] / parts[v]);
+      y[2 * (size_t)r + 1] = (float)std::log2(1.0 + csum[v]);
+    }
+    ++r;
+  }
+  return KCL_OK;
+}
+
+extern "C" kcl_status kcl_sim_replan(kcl_sim *s, uint32_t schedule, const kcl_cost_model *model) {
+  if (!s) return fail(KCL_E_INVALID_ARG, "NULL argument");
+  if (schedule > KCL_SCHED_DEGREE) return fail(KCL_E_INVALID_ARG, "unknown schedule %u", schedule);
+  s->opts.schedule = schedule;
+  s->opts.model = model;
+  if (s->trivial) return KCL_OK;
+  const auto t0 = std::chrono::steady_clock::now();
+  const uint32_t split = s->opts.split_work ? s->opts.split_work : (1u << 17);
+  kcl::PassPlan p;
+  KCL_TRY(kcl::plan_for_schedule(s->g, s->k, schedule, model, split, &s->root_cliques, s->stream,
+                                 &p));
+  KCL_TRY(kcl::upload_plan(p, &s->plan));
+  KCL_HIP(hipStreamSynchronize(s->stream));
+  s->info.tasks = (uint32_t)p.tasks.size();
+  s->info.plan_ms =
+      std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return KCL_OK;
 }
 
