@@ -33,9 +33,18 @@ WORKLOADS = {
              "Chung-Lu n=1e6 m=1.6e7 gamma=2.3 w_max=2e4 + 150-vertex community p=0.8, k=5"),
     "er20k": ("er20k", 1, 1, 4, "Erdos-Renyi n=2e4 m=1e5 + planted K30, k=4"),
     "cl1m_d16": ("cl1m", 1, 16, 5, "Chung-Lu cl1m scaled down 16x, k=5 (quick check)"),
+    "cl1m_k6": ("cl1m", 1, 1, 6,
+                "Chung-Lu n=1e6 m=1.6e7 gamma=2.3 w_max=2e4 + 150-vertex community p=0.8, k=6"),
     "cl4m": ("cl4m", 1, 1, 6,
              "Chung-Lu n=4e6 m=6e7 gamma=2.1 w_max=4e4 + 200-vertex near-clique p=0.9, k=6"),
 }
+
+
+def load_model(args, kcl):
+    """The cost model for --schedule learned (the shipped one by default)."""
+    if args.schedule != "learned":
+        return None
+    return kcl.CostModel(path=args.model or os.path.join(kcl.PKG, "cost_model.kclcm"))
 
 
 def log(*a):
@@ -79,31 +88,28 @@ def cpu_baseline(pairs: np.ndarray, k: int, target_s: float = 12.0):
     except Exception as e:  # pragma: no cover - reference build missing
         log("reference build unavailable:", e)
         return cpu_baseline_port(pairs, k, target_s)
-    import kclistpp_amd as kcl
-    with tempfile.TemporaryDirectory() as td:
-        path = os.path.join(td, "g.txt")
-        kcl.write_edge_list(path, pairs)
-        ref.threads(threads)
-        t0 = time.time()
-        h = ref.load(path)
-        log(f"cpu baseline: reference graph loaded in {time.time() - t0:.1f}s")
+    ref.threads(threads)
+    t0 = time.time()
+    h = ref.load_pairs(pairs)  # the reference's SortByCore/Relabel/MakeGraph
+    log(f"cpu baseline: reference graph built in {time.time() - t0:.1f}s")
     info = ref.info(h)
     ref.start(h, k)
     e = info["e"]
     rng = np.random.default_rng(12345)
-    frac = 0.002
+    chunk = 256
     spent, visits, tried = 0.0, 0.0, 0
     perm = rng.permutation(e).astype(np.uint32)
-    # Grow the sample until about target_s seconds of reference work.
+    # Grow the sample (chunks doubling from 256 root edges) until about
+    # target_s seconds of reference work.
     while spent < target_s and tried < e:
-        take = int(min(e - tried, max(1000, frac * e)))
+        take = int(min(e - tried, chunk))
         roots = np.sort(perm[tried:tried + take])
         s, v = ref.time_fw_roots(h, roots)
         spent += s
         visits += v
         tried += take
         log(f"cpu baseline: {tried} roots, {visits:.3e} visits, {spent:.1f}s")
-        frac *= 2
+        chunk *= 2
     ref.free(h)
     return {"value": visits / spent if spent > 0 else 0.0, "unit": "k-clique visits/s",
             "cores": threads, "kind": "reference",
@@ -125,6 +131,125 @@ def cpu_baseline_port(pairs, k, target_s):  # pragma: no cover - fallback only
     dt = time.time() - t0
     return {"value": v / dt, "unit": "k-clique visits/s", "cores": 1, "kind": "port",
             "sample": f"oracle restatement over {idx.size} random root edges, 1 thread, {dt:.1f}s"}
+
+
+_REF_GRAPH_CHILD = """
+import json, sys
+import numpy as np
+sys.path.insert(0, sys.argv[1])
+from oracle import RefHarness
+ref = RefHarness()
+ref.threads(int(sys.argv[4]))
+h = ref.load_pairs(np.load(sys.argv[2]))
+ref.start(h, int(sys.argv[3]))
+e = ref.info(h)["e"]
+s, v = ref.time_fw_roots(h, np.arange(e, dtype=np.uint32))
+print(json.dumps({"seconds": s, "visits": v, "edges": e}))
+"""
+
+
+def cpu_baseline_batch(pairs_list, ks, target_s: float = 12.0):
+    """The reference's CDF_CliqueEnum(FRANK_WOLFE) over whole graphs of the
+    batch (seeded order, all root edges, rho = 0), one child process per
+    graph (the reference sizes its per-thread scratch by the first graph it
+    sees), until about target_s seconds of reference work."""
+    import subprocess
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    order = np.random.default_rng(12345).permutation(len(pairs_list))
+    spent, visits, graphs = 0.0, 0.0, 0
+    with tempfile.TemporaryDirectory() as td:
+        for i in order:
+            if spent >= target_s:
+                break
+            path = os.path.join(td, "g.npy")
+            np.save(path, np.ascontiguousarray(pairs_list[i], np.uint32))
+            r = subprocess.run([sys.executable, "-c", _REF_GRAPH_CHILD, ROOT, path, str(ks[i]),
+                                str(threads)], capture_output=True, text=True, timeout=600)
+            if r.returncode != 0:
+                log(f"cpu baseline: reference failed on graph {i}: {r.stderr[-300:]}")
+                return None
+            res = json.loads([ln for ln in r.stdout.splitlines() if ln.startswith("{")][-1])
+            spent += res["seconds"]
+            visits += res["visits"]
+            graphs += 1
+            log(f"cpu baseline: graph {i} (k={ks[i]}, {res['edges']} edges): "
+                f"{res['visits']:.3e} visits in {res['seconds']:.2f}s")
+    return {"value": visits / spent if spent > 0 else 0.0, "unit": "k-clique visits/s",
+            "cores": threads, "kind": "reference",
+            "sample": f"CDF_CliqueEnum(FRANK_WOLFE) of the reference over every root edge of "
+                      f"{graphs} of the {len(pairs_list)} graphs (seeded order), rho = 0, "
+                      f"{threads} OpenMP threads, {spent:.1f}s"}
+
+
+def run_batch(args, kcl, gg):
+    """Mixed batch: 512 Chung-Lu graphs (n log-uniform in [1e3, 2e5], average
+    degree 4..40, gamma 2.1..2.8, k = 3 + (i mod 5)); a step is one
+    iteration of every graph (one hipGraph replay)."""
+    specs = list(gg.batch_specs(512, first_seed=1000))
+    t0 = time.time()
+    pairs_list, graphs = [], []
+    for i, sp in enumerate(specs):
+        p = gg.batch_graph(sp)
+        pairs_list.append(p)
+        graphs.append(kcl.Graph(p))
+        if i % 64 == 63:
+            log(f"bench batch: {i + 1} graphs generated and built ({time.time() - t0:.0f}s)")
+    ks = [sp["k"] for sp in specs]
+    model = load_model(args, kcl)
+    t1 = time.time()
+    b = kcl.Batch(graphs, ks, schedule=kcl.SCHEDULES[args.schedule], model=model,
+                  tune=not args.no_tune)
+    info = b.info()
+    log(f"bench batch: set-up {time.time() - t1:.1f}s: {info}")
+    for g in graphs:
+        g.close()
+    t = 0
+    for _ in range(args.warmup):
+        t += 1
+        b.step(t)
+        log(f"bench batch: warmup t={t} {b.last_pass()}")
+    visits = 0
+    kern_ms = 0.0
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        t += 1
+        b.step(t)
+        v, ms = b.last_pass()
+        if v != info["cliques"]:
+            raise RuntimeError(f"step {t} visited {v} cliques, the batch has {info['cliques']}")
+        visits += v
+        kern_ms += ms
+    elapsed = time.perf_counter() - t_start
+    log(f"bench batch: {args.steps} steps in {elapsed:.3f}s")
+    cpu = None if args.no_cpu_baseline else cpu_baseline_batch(pairs_list, ks)
+    b.close()
+    if model is not None:
+        model.close()
+    out = {
+        "metric": "k-clique visits/sec per kClist++ iteration",
+        "value": visits / elapsed if elapsed > 0 else 0.0,
+        "unit": "k-clique visits/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1000.0 * elapsed / max(args.steps, 1),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic (seeded generator, kclistpp_amd/graphgen.py)",
+        "config": {"workload": "batch512: 512 Chung-Lu graphs, n log-uniform [1e3, 2e5], avg "
+                               "degree U[4, 40], gamma U[2.1, 2.8], k = 3 + (i mod 5)",
+                   "graphs": 512, "nodes": info["nodes"], "edges": info["edges"],
+                   "cliques_per_step": info["cliques"], "schedule": args.schedule,
+                   "row_bits": info["row_bits"], "per_cu": info["per_cu"],
+                   "tasks": info["tasks"], "plan_ms": info["plan_ms"],
+                   "tune_ms": info["tune_ms"], "gpu_ms_per_step": kern_ms / max(args.steps, 1),
+                   "parallelism": "1 GPU"},
+        "roofline": None,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
 
 
 def run_exact(args, kcl, gg):
@@ -180,7 +305,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="cl1m", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="cl1m", choices=sorted(WORKLOADS) + ["batch512"])
+    ap.add_argument("--no-tune", action="store_true",
+                    help="batch512: keep the default row width and occupancy")
     ap.add_argument("--schedule", default="counted", choices=["counted", "learned", "degree"],
                     help="root order/splitting: counting-pass costs, the learned cost "
                          "model, or plain d+^2 (counts are identical under all three)")
@@ -202,6 +329,9 @@ def main():
 
     if args.mode == "exact":
         run_exact(args, kcl, gg)
+        return
+    if args.workload == "batch512":
+        run_batch(args, kcl, gg)
         return
     wname, seed, sd, k, desc = WORKLOADS[args.workload]
     log(f"bench: generating {args.workload} ({desc})")

@@ -281,6 +281,62 @@ kcl_status kcl_cmpsync_run(kcl_graph *g, uint32_t k, uint32_t num_iter, const kc
                            uint32_t *nrec);
 
 /* ---------------------------------------------------------------------
+ * Batched Sim-kClist++: many graphs, each with its own k, iterated
+ * together (no counterpart in the reference, which runs one graph per
+ * process; per graph the results are exactly those of kcl_sim /
+ * kcl_cmpsync_run on that graph alone).  The graphs are copied into one
+ * disjoint union in HBM; one iteration of every graph is one replay of a
+ * captured hipGraph (the enumeration launches of all distinct k plus one
+ * rho update).  The graph handles may be destroyed after create.
+ * ------------------------------------------------------------------- */
+typedef struct kcl_batch kcl_batch;
+
+typedef struct kcl_batch_opts {
+  uint32_t schedule;            /* kcl_schedule of every plan */
+  uint32_t split_work;          /* as kcl_sim_opts */
+  const kcl_cost_model *model;  /* for KCL_SCHED_LEARNED (nullable) */
+  uint32_t tune;      /* 1: pick the row-width floor and the resident
+                         workgroups per CU by timing one pass of each */
+  uint32_t row_bits;  /* tune = 0: bitmap row width floor 64/128/256 (0 = 64) */
+  uint32_t per_cu;    /* tune = 0: workgroups per CU cap (0 = LDS-limited) */
+  uint32_t reserved[3];
+} kcl_batch_opts;
+
+typedef struct kcl_batch_info {
+  uint32_t graphs;
+  uint32_t distinct_k;
+  uint64_t nodes;     /* relabelled nodes over all graphs */
+  uint64_t edges;
+  uint64_t cliques;   /* k-cliques over all graphs (each with its own k) */
+  uint32_t tasks;     /* workgroup tasks per iteration */
+  uint32_t row_bits;  /* chosen row-width floor */
+  uint32_t per_cu;    /* chosen cap (0: LDS-limited) */
+  uint32_t reserved;
+  double plan_ms;     /* union + plans */
+  double tune_ms;     /* autotuning */
+} kcl_batch_info;
+
+kcl_status kcl_batch_create(kcl_graph *const *graphs, const uint32_t *ks, uint32_t count,
+                            const kcl_batch_opts *opts, kcl_batch **out);
+void kcl_batch_destroy(kcl_batch *b);
+kcl_status kcl_batch_get_info(const kcl_batch *b, kcl_batch_info *info);
+/* k-cliques per graph (count entries). */
+kcl_status kcl_batch_graph_cliques(const kcl_batch *b, uint64_t *out);
+/* Iteration t >= 1 of every graph (one hipGraph replay). */
+kcl_status kcl_batch_step(kcl_batch *b, uint32_t t);
+/* Visits of the last iteration (all graphs) and its GPU time. */
+kcl_status kcl_batch_last_pass(const kcl_batch *b, uint64_t *visits, double *ms);
+/* Per graph i (n_i entries): last counts, rho, exact totals V. */
+kcl_status kcl_batch_counts(kcl_batch *b, uint32_t i, uint64_t *out);
+kcl_status kcl_batch_rho(kcl_batch *b, uint32_t i, double *out);
+kcl_status kcl_batch_totals(kcl_batch *b, uint32_t i, uint64_t *out);
+/* CDF_FindDensestSubset of every graph (count entries). */
+kcl_status kcl_batch_find_densest(kcl_batch *b, kcl_densest_info *out);
+/* CDF_Main of every graph: iterations up to the largest power of two <=
+ * num_iter, a record per power of two; records: count x 32, nrec: count. */
+kcl_status kcl_batch_run(kcl_batch *b, uint32_t num_iter, kcl_record *records, uint32_t *nrec);
+
+/* ---------------------------------------------------------------------
  * k-clique core trimming: kClistCoreTrim (kClistCoreTrim.c:707-775).
  * (k-1)-core by the reference's degree heap (nodes popped while the running
  * core is below k-1 are dropped; kept nodes get new ids n2-1, n2-2, ... in

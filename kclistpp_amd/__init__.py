@@ -22,7 +22,8 @@ from dataclasses import dataclass
 import numpy as np
 
 PKG = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(PKG, "libkclpp.so")
+# KCLPP_LIBRARY selects another build of the library (same ABI).
+LIB_PATH = os.environ.get("KCLPP_LIBRARY") or os.path.join(PKG, "libkclpp.so")
 
 FRANK_WOLFE = 2
 PAVA_PREPROCESS = 3
@@ -83,6 +84,19 @@ class _TrimInfo(C.Structure):
                 ("seconds_peel", C.c_double)]
 
 
+class _BatchOpts(C.Structure):
+    _fields_ = [("schedule", C.c_uint32), ("split_work", C.c_uint32), ("model", C.c_void_p),
+                ("tune", C.c_uint32), ("row_bits", C.c_uint32), ("per_cu", C.c_uint32),
+                ("reserved", C.c_uint32 * 3)]
+
+
+class _BatchInfo(C.Structure):
+    _fields_ = [("graphs", C.c_uint32), ("distinct_k", C.c_uint32), ("nodes", C.c_uint64),
+                ("edges", C.c_uint64), ("cliques", C.c_uint64), ("tasks", C.c_uint32),
+                ("row_bits", C.c_uint32), ("per_cu", C.c_uint32), ("reserved", C.c_uint32),
+                ("plan_ms", C.c_double), ("tune_ms", C.c_double)]
+
+
 class _ExactOpts(C.Structure):
     _fields_ = [("max_iter", C.c_uint32), ("max_seconds", C.c_uint32),
                 ("max_network_cliques", C.c_uint64), ("schedule", C.c_uint32),
@@ -120,7 +134,10 @@ EXPORTS = [
     "kcl_cost_model_create", "kcl_cost_model_destroy", "kcl_cost_model_load",
     "kcl_cost_model_save", "kcl_cost_model_get", "kcl_cost_model_set", "kcl_cost_model_predict",
     "kcl_cost_model_gradient", "kcl_cost_model_train", "kcl_sim_root_features",
-    "kcl_sim_cost_samples", "kcl_sim_replan",
+    "kcl_sim_cost_samples", "kcl_sim_replan", "kcl_batch_create", "kcl_batch_destroy",
+    "kcl_batch_get_info", "kcl_batch_graph_cliques", "kcl_batch_step", "kcl_batch_last_pass",
+    "kcl_batch_counts", "kcl_batch_rho", "kcl_batch_totals", "kcl_batch_find_densest",
+    "kcl_batch_run",
 ]
 
 
@@ -186,6 +203,17 @@ def lib() -> C.CDLL:
         "kcl_sim_root_features": ([vp, vp], C.c_int),
         "kcl_sim_cost_samples": ([vp, C.POINTER(u32), vp, vp, vp], C.c_int),
         "kcl_sim_replan": ([vp, u32, vp], C.c_int),
+        "kcl_batch_create": ([vp, vp, u32, C.POINTER(_BatchOpts), C.POINTER(vp)], C.c_int),
+        "kcl_batch_destroy": ([vp], None),
+        "kcl_batch_get_info": ([vp, C.POINTER(_BatchInfo)], C.c_int),
+        "kcl_batch_graph_cliques": ([vp, vp], C.c_int),
+        "kcl_batch_step": ([vp, u32], C.c_int),
+        "kcl_batch_last_pass": ([vp, C.POINTER(u64), C.POINTER(C.c_double)], C.c_int),
+        "kcl_batch_counts": ([vp, u32, vp], C.c_int),
+        "kcl_batch_rho": ([vp, u32, vp], C.c_int),
+        "kcl_batch_totals": ([vp, u32, vp], C.c_int),
+        "kcl_batch_find_densest": ([vp, C.POINTER(_Densest)], C.c_int),
+        "kcl_batch_run": ([vp, u32, C.POINTER(_Record), vp], C.c_int),
     }
     for name, (args, res) in sigs.items():
         f = getattr(L, name)
@@ -558,6 +586,100 @@ def cmpsync(graph: Graph, k: int, num_iter: int, stat_path: str | None = None,
                                  C.byref(nrec)))
     return [{"T": r.T, "n": r.n, "m": r.m, "density": r.density, "ub": r.ub, "sum": r.sum,
              "seconds": r.seconds} for r in recs[:nrec.value]]
+
+
+class Batch:
+    """Many graphs, each with its own k, iterated together (one captured
+    hipGraph replay per iteration of every graph)."""
+
+    def __init__(self, graphs: list[Graph], ks: list[int], schedule: int = SCHED_COUNTED,
+                 model: CostModel | None = None, tune: bool = False, row_bits: int = 0,
+                 per_cu: int = 0, split_work: int = 0):
+        if len(graphs) != len(ks) or not graphs:
+            raise ValueError("one k per graph, at least one graph")
+        self.graphs = list(graphs)
+        self.ks = [int(k) for k in ks]
+        self.n = [g.n for g in graphs]
+        o = _BatchOpts()
+        o.schedule, o.split_work, o.tune, o.row_bits, o.per_cu = (
+            schedule, split_work, int(tune), row_bits, per_cu)
+        o.model = model.handle if model is not None else None
+        hs = (C.c_void_p * len(graphs))(*[g.handle for g in graphs])
+        karr = np.ascontiguousarray(self.ks, np.uint32)
+        h = C.c_void_p()
+        _check(lib().kcl_batch_create(C.cast(hs, C.c_void_p), _ptr(karr), len(graphs),
+                                      C.byref(o), C.byref(h)))
+        self._h = h
+        self.t = 0
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().kcl_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def info(self) -> dict:
+        i = _BatchInfo()
+        _check(lib().kcl_batch_get_info(self._h, C.byref(i)))
+        return {f: getattr(i, f) for f, _ in _BatchInfo._fields_ if f != "reserved"}
+
+    def graph_cliques(self) -> np.ndarray:
+        out = np.zeros(len(self.graphs), np.uint64)
+        _check(lib().kcl_batch_graph_cliques(self._h, _ptr(out)))
+        return out
+
+    def step(self, t: int | None = None) -> None:
+        t = self.t + 1 if t is None else t
+        _check(lib().kcl_batch_step(self._h, t))
+        self.t = t
+
+    def last_pass(self) -> tuple[int, float]:
+        v = C.c_uint64(0)
+        ms = C.c_double(0)
+        _check(lib().kcl_batch_last_pass(self._h, C.byref(v), C.byref(ms)))
+        return int(v.value), float(ms.value)
+
+    def _seg(self, fn, i, dtype):
+        out = np.zeros(max(self.n[i], 1), dtype)
+        _check(fn(self._h, i, _ptr(out)))
+        return out[:self.n[i]]
+
+    def counts(self, i: int) -> np.ndarray:
+        return self._seg(lib().kcl_batch_counts, i, np.uint64)
+
+    def rho(self, i: int) -> np.ndarray:
+        return self._seg(lib().kcl_batch_rho, i, np.float64)
+
+    def totals(self, i: int) -> np.ndarray:
+        return self._seg(lib().kcl_batch_totals, i, np.uint64)
+
+    def find_densest(self) -> list[Densest]:
+        d = (_Densest * len(self.graphs))()
+        _check(lib().kcl_batch_find_densest(self._h, d))
+        return [Densest(x.n, x.m, x.density, x.ub, int(x.sum)) for x in d]
+
+    def run(self, num_iter: int) -> list[list[dict]]:
+        """CDF_Main of every graph: per graph, one record per power of two."""
+        recs = (_Record * (32 * len(self.graphs)))()
+        nrec = np.zeros(len(self.graphs), np.uint32)
+        _check(lib().kcl_batch_run(self._h, num_iter, recs, _ptr(nrec)))
+        out = []
+        for i in range(len(self.graphs)):
+            out.append([{"T": r.T, "n": r.n, "m": r.m, "density": r.density, "ub": r.ub,
+                         "sum": r.sum, "seconds": r.seconds}
+                        for r in recs[32 * i:32 * i + int(nrec[i])]])
+        return out
 
 
 class Trim:

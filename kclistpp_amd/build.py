@@ -35,7 +35,7 @@ HIP_FLAGS = [
     "-Wall", "-Wno-unused-function", "-I", INCLUDE, "-I", CSRC,
 ]
 
-SOURCES = ["host_io.cpp", "host_order.cpp", "graph.hip", "pass.hip", "plan.hip", "sim.hip",
+SOURCES = ["host_io.cpp", "host_order.cpp", "graph.hip", "pass.hip", "plan.hip", "sim.hip", "batch.hip",
            "costmodel.hip", "trim.cpp", "exact.cpp"]
 CLIS = {
     "kCDensestCmpSync": "cli_cmpsync.c",

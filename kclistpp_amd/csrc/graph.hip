@@ -12,6 +12,8 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
+#include <vector>
 
 #include "kcl_internal.h"
 
@@ -105,14 +107,26 @@ uint32_t grid_for(uint64_t work, uint32_t block) {
 }  // namespace
 
 kcl_status device_props(DeviceProps *out) {
+  // Queried once per device (launches ask for it, also during stream capture).
+  static std::mutex mu;
+  static std::vector<DeviceProps> cache;
   int dev = 0;
   KCL_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(mu);
+  for (const DeviceProps &c : cache)
+    if (c.device == dev) {
+      *out = c;
+      return KCL_OK;
+    }
   hipDeviceProp_t p;
   KCL_HIP(hipGetDeviceProperties(&p, dev));
-  out->device = dev;
-  out->cu_count = (uint32_t)p.multiProcessorCount;
-  out->lds_per_cu = (uint32_t)p.maxSharedMemoryPerMultiProcessor;
-  out->arch = p.gcnArchName;
+  DeviceProps d;
+  d.device = dev;
+  d.cu_count = (uint32_t)p.multiProcessorCount;
+  d.lds_per_cu = (uint32_t)p.maxSharedMemoryPerMultiProcessor;
+  d.arch = p.gcnArchName;
+  cache.push_back(d);
+  *out = d;
   return KCL_OK;
 }
 
@@ -252,6 +266,38 @@ kcl_status kcl::graph_create(const uint32_t *pairs, uint64_t m, uint32_t n_in,
   if (s != KCL_OK) {
     kcl_graph_destroy(g);
     return s;
+  }
+  *out = g;
+  return KCL_OK;
+}
+
+kcl_status kcl::graph_from_csr(std::vector<uint32_t> &&cd, std::vector<uint32_t> &&adj,
+                               kcl_graph **out) {
+  *out = nullptr;
+  if (cd.empty() || cd.back() != adj.size())
+    return fail(KCL_E_INVALID_ARG, "CSR offsets do not match the adjacency");
+  kcl_graph *g = new (std::nothrow) kcl_graph();
+  if (!g) return fail(KCL_E_OOM_HOST, "out of host memory");
+  g->n = (uint32_t)(cd.size() - 1);
+  g->n_in = g->n;
+  g->m = adj.size();
+  for (uint32_t v = 0; v < g->n; ++v) g->core = std::max(g->core, cd[v + 1] - cd[v]);
+  g->cd = std::move(cd);
+  g->adj = std::move(adj);
+  kcl_status st = KCL_OK;
+  hipError_t e = hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) st = fail(KCL_E_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
+  if (st == KCL_OK) st = g->d_cd.alloc(g->n + 1);
+  if (st == KCL_OK) st = g->d_adj.alloc(g->m);
+  if (st == KCL_OK) {
+    e = hipMemcpy(g->d_cd.p, g->cd.data(), (g->n + 1) * sizeof(uint32_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess && g->m)
+      e = hipMemcpy(g->d_adj.p, g->adj.data(), g->m * sizeof(uint32_t), hipMemcpyHostToDevice);
+    if (e != hipSuccess) st = fail(KCL_E_HIP, "hipMemcpy: %s", hipGetErrorString(e));
+  }
+  if (st != KCL_OK) {
+    kcl_graph_destroy(g);
+    return st;
   }
   *out = g;
   return KCL_OK;

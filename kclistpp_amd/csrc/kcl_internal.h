@@ -95,6 +95,7 @@ struct DevicePlan {
   DevBuf<uint32_t> tickets;  // one queue head per class
   uint32_t class_begin[kNumClasses + 1] = {};
   uint32_t class_count[kNumClasses] = {};
+  uint32_t max_per_cu = 0;  // cap on resident workgroups per CU (0: by LDS)
 };
 
 struct DeviceProps {
@@ -132,6 +133,9 @@ kcl_status build_plan(const std::vector<uint32_t> &cd, uint32_t k, const RootCos
                       uint32_t split_work, PassPlan *plan,
                       const std::vector<uint64_t> *keep = nullptr);
 kcl_status upload_plan(const PassPlan &plan, DevicePlan *dev);
+// Moves the tasks of classes below min_class into min_class (wider rows:
+// a class-c kernel handles any root with d+ <= 64 kClassWords[c]).
+void promote_plan(PassPlan *plan, uint32_t min_class);
 
 // Concurrent task slots of a word class in one launch (workgroups).
 uint32_t class_slots(uint32_t wclass);
@@ -143,7 +147,8 @@ uint32_t class_slots(uint32_t wclass);
 kcl_status make_plan(kcl_graph *g, uint32_t k, uint32_t split, hipStream_t st, PassPlan *out,
                      std::vector<uint64_t> *root_cliques, kcl_plan_info *info,
                      uint32_t schedule = KCL_SCHED_COUNTED,
-                     const kcl_cost_model *model = nullptr);
+                     const kcl_cost_model *model = nullptr,
+                     const std::vector<uint64_t> *candidates = nullptr);
 // Tasks for the roots with keep[s] != 0 under a schedule.
 kcl_status plan_for_schedule(kcl_graph *g, uint32_t k, uint32_t schedule,
                              const kcl_cost_model *model, uint32_t split,
@@ -162,6 +167,17 @@ kcl_status sim_level_pass(kcl_sim *s, bool full_key, std::vector<unsigned long l
 
 // Largest k with compiled enumeration kernels.
 uint32_t max_pass_k();
+
+// Densest prefix and upper bound of a report from host copies of tent (by
+// level), reordered and rho (kCDensestCmpSync.c:354-381); fills out's n,
+// density, ub and sum (not m) and best = n.
+kcl_status densest_host(const unsigned long long *tent, const uint32_t *reo, const double *rho,
+                        uint32_t n, uint32_t k, kcl_densest_info *out, uint32_t *best);
+
+// A graph from a ready DAG CSR (rows of smaller ids, ascending): no
+// relabelling, no core bitmap.  Takes the vectors.
+kcl_status graph_from_csr(std::vector<uint32_t> &&cd, std::vector<uint32_t> &&adj,
+                          kcl_graph **out);
 
 // Graph creation with the degeneracy heap (rank = nullptr) or given ranks
 // (a permutation of 0..n_in-1).

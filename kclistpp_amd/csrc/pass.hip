@@ -49,6 +49,14 @@ uint32_t blocks_per_cu(uint32_t wclass, const DeviceProps &props) {
 
 }  // namespace
 
+namespace {
+
+__global__ void zero_tickets_kernel(uint32_t *t) {
+  if (threadIdx.x < kNumClasses) t[threadIdx.x] = 0;
+}
+
+}  // namespace
+
 uint32_t class_slots(uint32_t wclass) {
   DeviceProps props;
   if (wclass >= kNumClasses || device_props(&props) != KCL_OK) return 1;
@@ -60,14 +68,18 @@ kcl_status launch_pass(const kcl_graph *g, const uint64_t *d_key, uint32_t k, Pa
                        unsigned long long *prof) {
   DeviceProps props;
   KCL_TRY(device_props(&props));
-  KCL_HIP(hipMemsetAsync(plan.tickets.p, 0, kNumClasses * sizeof(uint32_t), stream));
+  // A kernel rather than a memset node, so the pass can be graph-captured.
+  zero_tickets_kernel<<<1, 64, 0, stream>>>(plan.tickets.p);
+  KCL_HIP(hipGetLastError());
   for (int c = (int)kNumClasses - 1; c >= 0; --c) {  // heaviest word class first
     const uint32_t count = plan.class_count[c];
     if (!count) continue;
     PassFn fn = kernel_for((uint32_t)c, k, mode == kCreditAll);
     if (!fn) return fail(KCL_E_UNSUPPORTED, "no enumeration kernel for k = %u", k);
     const uint32_t waves = kClassWaves[c];
-    uint64_t blocks = (uint64_t)props.cu_count * blocks_per_cu((uint32_t)c, props);
+    uint32_t per_cu = blocks_per_cu((uint32_t)c, props);
+    if (plan.max_per_cu && plan.max_per_cu < per_cu) per_cu = plan.max_per_cu;
+    uint64_t blocks = (uint64_t)props.cu_count * per_cu;
     if (blocks > count) blocks = count;
     hipLaunchKernelGGL(fn, dim3((uint32_t)blocks), dim3(64 * waves), 0, stream,
                        (const uint32_t *)g->d_cd.p, (const uint32_t *)g->d_adj.p, d_key,
@@ -134,6 +146,29 @@ kcl_status build_plan(const std::vector<uint32_t> &cd, uint32_t k, const RootCos
   }
   plan->class_begin[kNumClasses] = at;
   return KCL_OK;
+}
+
+void promote_plan(PassPlan *plan, uint32_t min_class) {
+  if (min_class == 0 || min_class >= kNumClasses) return;
+  PassPlan out;
+  out.tasks.reserve(plan->tasks.size());
+  uint32_t at = 0;
+  for (uint32_t c = 0; c < kNumClasses; ++c) {
+    out.class_begin[c] = at;
+    if (c < min_class) continue;
+    // The floor class first (its own, heavier roots), then the promoted
+    // lighter classes, widest first.
+    for (uint32_t i = plan->class_begin[c]; i < plan->class_begin[c + 1]; ++i)
+      out.tasks.push_back(plan->tasks[i]);
+    if (c == min_class)
+      for (uint32_t l = min_class; l-- > 0;)
+        for (uint32_t i = plan->class_begin[l]; i < plan->class_begin[l + 1]; ++i)
+          out.tasks.push_back(plan->tasks[i]);
+    out.class_count[c] = (uint32_t)out.tasks.size() - at;
+    at = (uint32_t)out.tasks.size();
+  }
+  out.class_begin[kNumClasses] = at;
+  *plan = std::move(out);
 }
 
 kcl_status upload_plan(const PassPlan &plan, DevicePlan *dev) {

@@ -54,6 +54,7 @@ struct Lds {
   uint32_t pre[D + 1];          // prefix sums (member degrees, later |A'[u]|)
   unsigned long long root_sum;  // the root's credits, summed over the waves
   uint32_t task;                // ticket broadcast
+  uint32_t next_item;           // counting items handed out so far
 };
 
 __device__ __forceinline__ uint32_t select_bit(unsigned long long x, uint32_t r) {
@@ -376,8 +377,9 @@ __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restric
         }
         const uint32_t ilo = (uint32_t)((uint64_t)part * nitems / np);
         const uint32_t ihi = (uint32_t)((uint64_t)(part + 1) * nitems / np);
+        if (tid == 0) sh.next_item = ilo;
+        __syncthreads();
 
-        uint32_t e = ilo + tid;
         uint32_t u = 0, vsec = 0, wi = 0;
         unsigned long long P[W];
         unsigned long long bits = 0;
@@ -385,12 +387,31 @@ __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restric
         bool live = false;
         uint32_t ucur = 0;
 
-        // Next item with |P| >= kMinSet, or live = false.
+        // Next item with |P| >= kMinSet, or live = false.  For k <= 5 items
+        // are handed out on demand (one LDS atomic per wavefront and round,
+        // each lane taking the next index), so a lane that drew light items
+        // keeps drawing while another works through a heavy one; for k >= 6,
+        // where an item's own inner loops dominate, lanes take every NT-th
+        // item (measured faster there).  A lane's items are increasing,
+        // which the search for u below relies on.
+        constexpr bool kOnDemand = (K <= 5);
+        uint32_t e = ilo + tid;
         auto fetch = [&]() {
           live = false;
-          while (e < ihi) {
-            const uint32_t it = e;
-            e += NT;
+          for (;;) {
+            uint32_t it;
+            if constexpr (kOnDemand) {
+              const unsigned long long act = __ballot(1);
+              const uint32_t leader = (uint32_t)__builtin_ctzll(act);
+              uint32_t base = 0;
+              if (lane == leader) base = atomicAdd(&sh.next_item, (uint32_t)__popcll(act));
+              base = __shfl(base, (int)leader);
+              it = base + (uint32_t)__popcll(act & ((1ull << lane) - 1ull));
+            } else {
+              it = e;
+              e += NT;
+            }
+            if (it >= ihi) return;
             if constexpr (kEdgeItems) {
               uint32_t lo = ucur, hi = d;  // pre[u] <= it < pre[u+1]
               while (hi - lo > 1) {

@@ -220,7 +220,8 @@ kcl_status plan_for_schedule(kcl_graph *g, uint32_t k, uint32_t schedule,
 
 kcl_status make_plan(kcl_graph *g, uint32_t k, uint32_t split, hipStream_t st, PassPlan *out,
                      std::vector<uint64_t> *root_cliques, kcl_plan_info *info,
-                     uint32_t schedule, const kcl_cost_model *model) {
+                     uint32_t schedule, const kcl_cost_model *model,
+                     const std::vector<uint64_t> *candidates) {
   if (schedule > KCL_SCHED_DEGREE)
     return fail(KCL_E_INVALID_ARG, "unknown schedule %u", schedule);
   const auto t0 = std::chrono::steady_clock::now();
@@ -230,9 +231,9 @@ kcl_status make_plan(kcl_graph *g, uint32_t k, uint32_t split, hipStream_t st, P
   std::vector<float> pred;
   if (learned) {
     KCL_TRY(learned_predictions(g, k, model, st, &pred));
-    KCL_TRY(build_plan_learned(g->cd, k, pred, nullptr, &all));
+    KCL_TRY(build_plan_learned(g->cd, k, pred, candidates, &all));
   } else {
-    KCL_TRY(plan_for_schedule(g, k, schedule, model, split, nullptr, st, &all));
+    KCL_TRY(plan_for_schedule(g, k, schedule, model, split, candidates, st, &all));
   }
   std::vector<uint64_t> ck;
   KCL_TRY(root_counts(g, k, all, st, &ck));
@@ -260,7 +261,8 @@ kcl_status make_plan(kcl_graph *g, uint32_t k, uint32_t split, hipStream_t st, P
     for (uint32_t v = 0; v < n; ++v) {
       cliques += ck[v];
       kept += ck[v] ? 1u : 0u;
-      roots_all += (g->cd[v + 1] - g->cd[v] >= k - 1) ? 1u : 0u;
+      roots_all += (g->cd[v + 1] - g->cd[v] >= k - 1 && (!candidates || (*candidates)[v]))
+                       ? 1u : 0u;
     }
     info->roots_total = roots_all;
     info->roots_kept = kept;

@@ -285,22 +285,8 @@ kcl_status sim_level_pass(kcl_sim *s, bool full_key, std::vector<unsigned long l
   return KCL_OK;
 }
 
-kcl_status sim_find_densest(kcl_sim *s, kcl_densest_info *out, uint32_t *h_reordered,
-                            uint32_t *h_level, uint64_t *h_tent) {
-  kcl_graph *g = s->g;
-  const uint32_t n = g->n;
-  hipStream_t st = s->stream;
-  out->n = 0;
-  out->m = 0;
-  out->density = -1;
-  out->ub = 0;
-  out->sum = 0;
-  if (n == 0) return KCL_OK;  // the reference reads an unset n here
-  std::vector<unsigned long long> tent;
-  std::vector<uint32_t> reo;
-  std::vector<double> rho;
-  KCL_TRY(sim_level_pass(s, false, tent, reo, rho));
-
+kcl_status densest_host(const unsigned long long *tent, const uint32_t *reo, const double *rho,
+                        uint32_t n, uint32_t k, kcl_densest_info *out, uint32_t *best_out) {
   // Densest prefix (kCDensestCmpSync.c:354-361): the running sum is a sum
   // of integers below 2^53, so the double is exact and matches.
   double sum = 0, density = -1;
@@ -316,22 +302,8 @@ kcl_status sim_find_densest(kcl_sim *s, kcl_densest_info *out, uint32_t *h_reord
     }
   }
   if (isum >= kExactLimit) return fail(KCL_E_OVERFLOW, "%llu cliques: beyond 2^53", isum);
-  // CountEdges over the first `best` nodes of reordered.
-  {
-    DevBuf<unsigned long long> d_m;
-    KCL_TRY(d_m.alloc(1));
-    KCL_HIP(hipMemsetAsync(d_m.p, 0, 8, st));
-    count_top_edges_kernel<<<grid_for(n, 256), 256, 0, st>>>(g->d_cd.p, g->d_adj.p, s->level.p, n,
-                                                             best, d_m.p);
-    KCL_HIP(hipGetLastError());
-    unsigned long long mm = 0;
-    KCL_HIP(hipMemcpyAsync(&mm, d_m.p, 8, hipMemcpyDeviceToHost, st));
-    KCL_HIP(hipStreamSynchronize(st));
-    out->m = (uint32_t)mm;
-  }
   // Upper bound (kCDensestCmpSync.c:365-381), sequential in level order.
   double s2 = 0, ub = 0, ip1ck = 1;
-  const uint32_t k = s->k;
   for (uint32_t i = 0; i < n; ++i) {
     s2 += rho[reo[i]];
     if (i + 1 == k)
@@ -349,6 +321,41 @@ kcl_status sim_find_densest(kcl_sim *s, kcl_densest_info *out, uint32_t *h_reord
   out->density = density;
   out->ub = ub;
   out->sum = best_sum;
+  *best_out = best;
+  return KCL_OK;
+}
+
+kcl_status sim_find_densest(kcl_sim *s, kcl_densest_info *out, uint32_t *h_reordered,
+                            uint32_t *h_level, uint64_t *h_tent) {
+  kcl_graph *g = s->g;
+  const uint32_t n = g->n;
+  hipStream_t st = s->stream;
+  out->n = 0;
+  out->m = 0;
+  out->density = -1;
+  out->ub = 0;
+  out->sum = 0;
+  if (n == 0) return KCL_OK;  // the reference reads an unset n here
+  std::vector<unsigned long long> tent;
+  std::vector<uint32_t> reo;
+  std::vector<double> rho;
+  KCL_TRY(sim_level_pass(s, false, tent, reo, rho));
+
+  uint32_t best = 0;
+  KCL_TRY(densest_host(tent.data(), reo.data(), rho.data(), n, s->k, out, &best));
+  // CountEdges over the first `best` nodes of reordered.
+  {
+    DevBuf<unsigned long long> d_m;
+    KCL_TRY(d_m.alloc(1));
+    KCL_HIP(hipMemsetAsync(d_m.p, 0, 8, st));
+    count_top_edges_kernel<<<grid_for(n, 256), 256, 0, st>>>(g->d_cd.p, g->d_adj.p, s->level.p, n,
+                                                             best, d_m.p);
+    KCL_HIP(hipGetLastError());
+    unsigned long long mm = 0;
+    KCL_HIP(hipMemcpyAsync(&mm, d_m.p, 8, hipMemcpyDeviceToHost, st));
+    KCL_HIP(hipStreamSynchronize(st));
+    out->m = (uint32_t)mm;
+  }
   if (h_reordered) memcpy(h_reordered, reo.data(), (size_t)n * 4);
   if (h_tent) memcpy(h_tent, tent.data(), (size_t)n * 8);
   if (h_level) {

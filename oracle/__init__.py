@@ -207,6 +207,8 @@ class RefHarness:
         L = _load(path)
         L.kclref_load.argtypes = [C.c_char_p]
         L.kclref_load.restype = C.c_void_p
+        L.kclref_load_pairs.argtypes = [_u32p, C.c_uint]
+        L.kclref_load_pairs.restype = C.c_void_p
         for f in ("kclref_n_in", "kclref_n", "kclref_e", "kclref_core"):
             getattr(L, f).argtypes = [C.c_void_p]
             getattr(L, f).restype = C.c_uint
@@ -226,6 +228,15 @@ class RefHarness:
 
     def load(self, path: str):
         h = self.L.kclref_load(path.encode())
+        if not h:
+            raise RuntimeError("reference load failed")
+        return h
+
+    def load_pairs(self, pairs: np.ndarray):
+        """The reference's graph from edge pairs in memory (same EdgeList as
+        its reader would build from their text)."""
+        p = np.ascontiguousarray(pairs, np.uint32)
+        h = self.L.kclref_load_pairs(p, len(p))
         if not h:
             raise RuntimeError("reference load failed")
         return h

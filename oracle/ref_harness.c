@@ -58,6 +58,31 @@ void *kclref_load(const char *path) {
   return h;
 }
 
+/* The same graph from uint32 pairs in memory: the EdgeList ReadEdgeList
+ * would build from their text (n = max id + 1, Graph.c:34-53), then the
+ * reference's own SortByCore / Relabel / MakeGraph. */
+void *kclref_load_pairs(const unsigned *pairs, unsigned m) {
+  EdgeList *el = (EdgeList *)malloc(sizeof(EdgeList));
+  el->e = m;
+  el->n = 0;
+  el->edges = (Edge *)malloc((m ? m : 1) * sizeof(Edge));
+  for (unsigned i = 0; i < m; ++i) {
+    el->edges[i].s = pairs[2 * (size_t)i];
+    el->edges[i].t = pairs[2 * (size_t)i + 1];
+    if (el->edges[i].s > el->n) el->n = el->edges[i].s;
+    if (el->edges[i].t > el->n) el->n = el->edges[i].t;
+  }
+  ++el->n;
+  kclref *h = (kclref *)calloc(1, sizeof(kclref));
+  h->n_in = el->n;
+  SortByCore(el);
+  h->rank = (unsigned *)malloc(el->n * sizeof(unsigned));
+  memcpy(h->rank, el->rank, el->n * sizeof(unsigned));
+  Relabel(el);
+  h->g = MakeGraph(el);
+  return h;
+}
+
 unsigned kclref_n_in(void *p) { return ((kclref *)p)->n_in; }
 unsigned kclref_n(void *p) { return ((kclref *)p)->g->n; }
 unsigned kclref_e(void *p) { return ((kclref *)p)->g->e; }

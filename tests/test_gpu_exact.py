@@ -155,3 +155,19 @@ def test_exact_cli_stat_file(kcl, tmp_path):
     f = lines[-1].split("\t")
     assert f[1] == "Max Flow" and int(f[2]) == 12 and int(f[3]) == 66
     assert f[4] == "%.12f" % (math.comb(12, 4) / 12)
+
+
+def test_exact_over_budget_reports_bounds(kcl):
+    """With a network budget too small for any certificate the result is
+    the densest subgraph found, uncertified, with a valid upper bound."""
+    rng = np.random.default_rng(9)
+    pairs, _ = gg.erdos_renyi(400, 2400, 9)
+    pairs = gg.plant_clique(pairs, 400, 14, rng)
+    with kcl.Graph(pairs) as g:
+        r_small, _ = kcl.exact(g, 4, max_network_cliques=10, max_iter=8)
+        r_full, nodes = kcl.exact(g, 4)
+    assert r_small["certified"] == 0 and r_small["max_flow_calls"] == 0
+    assert r_small["upper_bound"] >= r_small["density"] > 0
+    assert r_full["certified"] == 1
+    assert r_small["density"] <= r_full["density"] <= r_small["upper_bound"] + 1e-9
+    assert r_full["n"] == len(nodes)

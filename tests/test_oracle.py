@@ -237,3 +237,22 @@ def test_oracle_max_flow(oracle, seed):
             G.add_edge(a, b, capacity=c)
     want = nx.maximum_flow_value(G, 0, nv - 1) if G.has_node(0) and G.has_node(nv - 1) else 0
     assert oracle.max_flow(nv, frm, to, cap, 0, nv - 1) == want
+
+
+def test_reference_pairs_loader_matches_its_reader(tmp_path):
+    """oracle/ref_harness.c's in-memory loader builds the same graph as the
+    reference's own text reader (used by bench.py's CPU baseline)."""
+    import kclistpp_amd as kcl
+    from kclistpp_amd import graphgen as gg
+    from oracle import RefHarness
+    try:
+        ref = RefHarness()
+    except OSError:
+        pytest.skip("reference build (oracle/_ref) absent")
+    pairs = gg.workload("er20k", seed=4)
+    path = str(tmp_path / "g.txt")
+    kcl.write_edge_list(path, pairs)
+    a, b = ref.load(path), ref.load_pairs(pairs)
+    assert ref.info(a) == ref.info(b)
+    assert np.array_equal(ref.rank(a), ref.rank(b))
+    assert np.array_equal(ref.edges(a), ref.edges(b))
