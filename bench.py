@@ -148,18 +148,20 @@ print(json.dumps({"seconds": s, "visits": v, "edges": e}))
 """
 
 
-def cpu_baseline_batch(pairs_list, ks, target_s: float = 12.0):
+def cpu_baseline_batch(pairs_list, ks, target_s: float = 12.0, wall_s: float = 90.0):
     """The reference's CDF_CliqueEnum(FRANK_WOLFE) over whole graphs of the
     batch (seeded order, all root edges, rho = 0), one child process per
     graph (the reference sizes its per-thread scratch by the first graph it
-    sees), until about target_s seconds of reference work."""
+    sees), until about target_s seconds of reference work or wall_s seconds
+    of wall clock (loading each graph dominates for these sparse graphs)."""
     import subprocess
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     order = np.random.default_rng(12345).permutation(len(pairs_list))
     spent, visits, graphs = 0.0, 0.0, 0
+    w0 = time.time()
     with tempfile.TemporaryDirectory() as td:
         for i in order:
-            if spent >= target_s:
+            if spent >= target_s or time.time() - w0 > wall_s:
                 break
             path = os.path.join(td, "g.npy")
             np.save(path, np.ascontiguousarray(pairs_list[i], np.uint32))
