@@ -94,6 +94,13 @@ kcl_status kcl_graph_rank(const kcl_graph *g, uint32_t *rank);
 kcl_status kcl_graph_csr(const kcl_graph *g, uint32_t *cd, uint32_t *adj);
 /* st[2m]: edges sorted by (s, t) (Graph.c:125). */
 kcl_status kcl_graph_edges(const kcl_graph *g, uint32_t *st);
+/* Roots with more DAG out-neighbours than max_vertex_degree (1..1024,
+ * default 1024) are enumerated as edge tasks: one task per out-edge (s, u)
+ * over N+(s) & N+(u), the reference's own root unit (kCDensestCmpSync.c:
+ * 300-305).  Above 1024 there is no vertex task, so this is also how graphs
+ * whose degeneracy exceeds 1024 run (each common out-neighbourhood must stay
+ * <= 1024).  Changes timings only; plans made after the call use it. */
+kcl_status kcl_graph_set_edge_split(kcl_graph *g, uint32_t max_vertex_degree);
 /* CountEdges (Graph.h:51, Graph.c:158-166): edges with both ends in nodes[]. */
 kcl_status kcl_count_edges(const kcl_graph *g, uint32_t num_nodes, const uint32_t *nodes,
                            uint32_t *m_out);

@@ -126,6 +126,7 @@ _lib = None
 EXPORTS = [
     "kcl_last_error", "kcl_status_name", "kcl_read_edge_list", "kcl_write_edge_list", "kcl_free",
     "kcl_graph_create", "kcl_graph_destroy", "kcl_graph_get_info", "kcl_graph_rank",
+    "kcl_graph_set_edge_split",
     "kcl_graph_csr", "kcl_graph_edges", "kcl_count_edges", "kcl_clique_enum", "kcl_sim_create",
     "kcl_sim_destroy", "kcl_sim_step", "kcl_sim_counts", "kcl_sim_rho", "kcl_sim_set_rho",
     "kcl_sim_totals", "kcl_sim_last_pass", "kcl_sim_plan_info", "kcl_sim_root_cliques", "kcl_sim_profile_pass", "kcl_sim_find_densest", "kcl_cmpsync_run",
@@ -162,6 +163,7 @@ def lib() -> C.CDLL:
         "kcl_graph_destroy": ([vp], None),
         "kcl_graph_get_info": ([vp, C.POINTER(_GraphInfo)], C.c_int),
         "kcl_graph_rank": ([vp, vp], C.c_int),
+        "kcl_graph_set_edge_split": ([vp, u32], C.c_int),
         "kcl_graph_csr": ([vp, vp, vp], C.c_int),
         "kcl_graph_edges": ([vp, vp], C.c_int),
         "kcl_count_edges": ([vp, u32, vp, C.POINTER(u32)], C.c_int),
@@ -331,6 +333,10 @@ class Graph:
         st = np.zeros(max(2 * self.m, 2), np.uint32)
         _check(lib().kcl_graph_edges(self.handle, _ptr(st)))
         return st[:2 * self.m].reshape(-1, 2)
+
+    def set_edge_split(self, max_vertex_degree: int) -> None:
+        """Roots with more DAG out-neighbours run as edge tasks (timing only)."""
+        _check(lib().kcl_graph_set_edge_split(self.handle, max_vertex_degree))
 
     def count_edges(self, nodes: np.ndarray) -> int:
         nodes = np.ascontiguousarray(nodes, dtype=np.uint32)

@@ -320,6 +320,7 @@ kcl_status batch_create(kcl_graph *const *graphs, const uint32_t *ks, uint32_t c
     cd[N] = (uint32_t)M;
   }
   KCL_TRY(graph_from_csr(std::move(cd), std::move(adj), &b->u));
+  for (uint32_t i = 0; i < count; ++i) b->u->edge_above = std::min(b->u->edge_above, graphs[i]->edge_above);
   hipStream_t st = b->u->stream;
   for (int i = 0; i < 2; ++i) KCL_HIP(hipEventCreate(&b->ev[i]));
   KCL_TRY(b->rho.alloc(N));
@@ -366,7 +367,7 @@ kcl_status batch_create(kcl_graph *const *graphs, const uint32_t *ks, uint32_t c
     KCL_TRY(make_plan(b->u, k, split, st, &kp->host, &ck, nullptr, o.schedule, o.model, &cand));
     for (uint32_t v = 0; v < N; ++v)
       if (ck[v]) b->cliques[gid[v]] += ck[v];
-    tasks += (uint32_t)kp->host.tasks.size();
+    tasks += (uint32_t)(kp->host.tasks.size() + kp->host.edge_tasks.size());
     b->plans.push_back(std::move(kp));
   }
   const auto t1 = std::chrono::steady_clock::now();

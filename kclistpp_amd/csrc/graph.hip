@@ -311,6 +311,15 @@ extern "C" void kcl_graph_destroy(kcl_graph *g) {
   delete g;
 }
 
+extern "C" kcl_status kcl_graph_set_edge_split(kcl_graph *g, uint32_t max_vertex_degree) {
+  if (!g) return fail(KCL_E_INVALID_ARG, "NULL argument");
+  if (max_vertex_degree < 1 || max_vertex_degree > kcl::kMaxRootDegree)
+    return fail(KCL_E_INVALID_ARG, "max_vertex_degree = %u: 1 <= value <= %u", max_vertex_degree,
+                kcl::kMaxRootDegree);
+  g->edge_above = max_vertex_degree;
+  return KCL_OK;
+}
+
 extern "C" kcl_status kcl_graph_get_info(const kcl_graph *g, kcl_graph_info *info) {
   if (!g || !info) return fail(KCL_E_INVALID_ARG, "NULL argument");
   info->n_in = g->n_in;

@@ -70,11 +70,19 @@ constexpr uint32_t kClassWaves[kNumClasses] = {1, 2, 4, 4, 4, 4, 4, 4};
 constexpr uint32_t kMaxWords = 16;
 constexpr uint32_t kMaxRootDegree = 64 * kMaxWords;
 
-// One root task: root node s, split part p of np (items [p*I/np, (p+1)*I/np)).
+// One root task: root node s, split part p of np (items [p*I/np, (p+1)*I/np)),
+// or an edge task (root edge (s, u), u < 2^31, flagged in the top bit).
 struct Task {
   uint32_t s;
-  uint32_t part_np;  // part | (np << 16)
+  uint32_t part_np;  // part | (np << 16), or kEdgeTaskFlag | u
 };
+constexpr uint32_t kEdgeTaskFlag = 0x80000000u;
+__host__ __device__ inline bool task_is_edge(uint32_t part_np) {
+  return (part_np & kEdgeTaskFlag) != 0;
+}
+__host__ __device__ inline uint32_t task_edge_node(uint32_t part_np) {
+  return part_np & ~kEdgeTaskFlag;
+}
 
 enum PassMode : uint32_t {
   kCreditWinner = 0,  // each clique credits its minimum-key member
@@ -83,18 +91,25 @@ enum PassMode : uint32_t {
 };
 
 struct PassPlan {
-  // Tasks grouped by word class, heaviest first inside each class.
+  // Vertex tasks grouped by word class, heaviest first inside each class.
   std::vector<Task> tasks;
   uint32_t class_begin[kNumClasses + 1] = {};
   uint32_t class_count[kNumClasses] = {};
+  // Edge tasks of the roots with d+ > kMaxRootDegree, grouped by the word
+  // class of their member count (run by the (k-1) kernels).
+  std::vector<Task> edge_tasks;
+  uint32_t eclass_begin[kNumClasses + 1] = {};
+  uint32_t eclass_count[kNumClasses] = {};
 };
 
 struct DevicePlan {
-  std::vector<Task> host_tasks;  // same order as tasks
+  std::vector<Task> host_tasks;  // vertex tasks, then edge tasks
   DevBuf<Task> tasks;
-  DevBuf<uint32_t> tickets;  // one queue head per class
+  DevBuf<uint32_t> tickets;  // one queue head per class, vertex then edge
   uint32_t class_begin[kNumClasses + 1] = {};
   uint32_t class_count[kNumClasses] = {};
+  uint32_t eclass_begin[kNumClasses + 1] = {};  // offsets into tasks
+  uint32_t eclass_count[kNumClasses] = {};
   uint32_t max_per_cu = 0;  // cap on resident workgroups per CU (0: by LDS)
 };
 
@@ -128,10 +143,14 @@ struct RootCost {
 constexpr uint64_t kBuildShare = 8;
 
 // Tasks for every root with d+(s) >= k-1 (rc nullable: d^2 estimate).
-// keep (nullable): only roots with keep[s] != 0 get tasks.
-kcl_status build_plan(const std::vector<uint32_t> &cd, uint32_t k, const RootCost *rc,
-                      uint32_t split_work, PassPlan *plan,
-                      const std::vector<uint64_t> *keep = nullptr);
+// keep (nullable): only roots with keep[s] != 0 get tasks.  Roots above
+// kMaxRootDegree become edge tasks (their sizes counted on the GPU).
+kcl_status build_plan(const kcl_graph *g, uint32_t k, const RootCost *rc, uint32_t split_work,
+                      PassPlan *plan, const std::vector<uint64_t> *keep = nullptr);
+// Edge tasks for the roots in `big` (every out-edge (s, u) whose common
+// out-neighbourhood can hold a (k-2)-clique), grouped by class.
+kcl_status add_edge_tasks(const kcl_graph *g, uint32_t k, const std::vector<uint32_t> &big,
+                          PassPlan *plan);
 kcl_status upload_plan(const PassPlan &plan, DevicePlan *dev);
 // Moves the tasks of classes below min_class into min_class (wider rows:
 // a class-c kernel handles any root with d+ <= 64 kClassWords[c]).
@@ -202,5 +221,7 @@ struct kcl_graph {
   // part of the degeneracy order): row v, bit u set iff {u, v} is an edge.
   kcl::DevBuf<unsigned long long> d_core;
   uint32_t core_h = 0;
+  // Roots with more out-neighbours than this run as edge tasks.
+  uint32_t edge_above = kcl::kMaxRootDegree;
   hipStream_t stream = nullptr;
 };

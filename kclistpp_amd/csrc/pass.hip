@@ -52,7 +52,7 @@ uint32_t blocks_per_cu(uint32_t wclass, const DeviceProps &props) {
 namespace {
 
 __global__ void zero_tickets_kernel(uint32_t *t) {
-  if (threadIdx.x < kNumClasses) t[threadIdx.x] = 0;
+  if (threadIdx.x < 2 * kNumClasses) t[threadIdx.x] = 0;
 }
 
 }  // namespace
@@ -69,32 +69,139 @@ kcl_status launch_pass(const kcl_graph *g, const uint64_t *d_key, uint32_t k, Pa
   DeviceProps props;
   KCL_TRY(device_props(&props));
   // A kernel rather than a memset node, so the pass can be graph-captured.
+  static_assert(2 * kNumClasses <= 64, "tickets of one pass fit one wavefront");
   zero_tickets_kernel<<<1, 64, 0, stream>>>(plan.tickets.p);
   KCL_HIP(hipGetLastError());
-  for (int c = (int)kNumClasses - 1; c >= 0; --c) {  // heaviest word class first
-    const uint32_t count = plan.class_count[c];
-    if (!count) continue;
-    PassFn fn = kernel_for((uint32_t)c, k, mode == kCreditAll);
-    if (!fn) return fail(KCL_E_UNSUPPORTED, "no enumeration kernel for k = %u", k);
-    const uint32_t waves = kClassWaves[c];
-    uint32_t per_cu = blocks_per_cu((uint32_t)c, props);
-    if (plan.max_per_cu && plan.max_per_cu < per_cu) per_cu = plan.max_per_cu;
-    uint64_t blocks = (uint64_t)props.cu_count * per_cu;
-    if (blocks > count) blocks = count;
-    hipLaunchKernelGGL(fn, dim3((uint32_t)blocks), dim3(64 * waves), 0, stream,
-                       (const uint32_t *)g->d_cd.p, (const uint32_t *)g->d_adj.p, d_key,
-                       (const unsigned long long *)(g->core_h ? g->d_core.p : nullptr), g->core_h,
-                       plan.tasks.p + plan.class_begin[c], count, plan.tickets.p + c,
-                       (uint32_t)mode, d_cnt,
-                       prof ? prof + 2 * (size_t)plan.class_begin[c] : nullptr);
-    KCL_HIP(hipGetLastError());
+  // Vertex tasks with the k kernels, then edge tasks with the (k-1) ones.
+  for (uint32_t grp = 0; grp < 2; ++grp) {
+    const uint32_t *begin = grp ? plan.eclass_begin : plan.class_begin;
+    const uint32_t *cnts = grp ? plan.eclass_count : plan.class_count;
+    const uint32_t kk = grp ? k - 1 : k;
+    for (int c = (int)kNumClasses - 1; c >= 0; --c) {  // heaviest word class first
+      const uint32_t count = cnts[c];
+      if (!count) continue;
+      PassFn fn = kernel_for((uint32_t)c, kk, mode == kCreditAll);
+      if (!fn) return fail(KCL_E_UNSUPPORTED, "no enumeration kernel for k = %u", kk);
+      const uint32_t waves = kClassWaves[c];
+      uint32_t per_cu = blocks_per_cu((uint32_t)c, props);
+      if (plan.max_per_cu && plan.max_per_cu < per_cu) per_cu = plan.max_per_cu;
+      uint64_t blocks = (uint64_t)props.cu_count * per_cu;
+      if (blocks > count) blocks = count;
+      hipLaunchKernelGGL(fn, dim3((uint32_t)blocks), dim3(64 * waves), 0, stream,
+                         (const uint32_t *)g->d_cd.p, (const uint32_t *)g->d_adj.p, d_key,
+                         (const unsigned long long *)(g->core_h ? g->d_core.p : nullptr),
+                         g->core_h, plan.tasks.p + begin[c], count,
+                         plan.tickets.p + grp * kNumClasses + c, (uint32_t)mode, d_cnt,
+                         prof ? prof + 2 * (size_t)begin[c] : nullptr);
+      KCL_HIP(hipGetLastError());
+    }
   }
   return KCL_OK;
 }
 
-kcl_status build_plan(const std::vector<uint32_t> &cd, uint32_t k, const RootCost *rc,
-                      uint32_t split_work, PassPlan *plan, const std::vector<uint64_t> *keep) {
+namespace {
+
+// Members of N+(s) & N+(u) per edge task (one wavefront per task: lanes
+// walk the shorter list and search the longer).
+__global__ void edge_members_kernel(const uint32_t *__restrict__ cd,
+                                    const uint32_t *__restrict__ adj,
+                                    const Task *__restrict__ tasks, uint32_t n,
+                                    uint32_t *__restrict__ out) {
+  const uint32_t lane = threadIdx.x & 63;
+  for (uint64_t t = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; t < n;
+       t += ((uint64_t)gridDim.x * blockDim.x) >> 6) {
+    const uint32_t s = tasks[t].s, u = task_edge_node(tasks[t].part_np);
+    const uint32_t bs = cd[s], ls = cd[s + 1] - bs, bu = cd[u], lu = cd[u + 1] - bu;
+    const uint32_t *A = adj + (ls <= lu ? bs : bu);
+    const uint32_t *B = adj + (ls <= lu ? bu : bs);
+    const uint32_t la = ls <= lu ? ls : lu, lb = ls <= lu ? lu : ls;
+    uint32_t hits = 0;
+    for (uint32_t i = lane; i < la; i += 64) {
+      const uint32_t x = A[i];
+      uint32_t lo = 0, hi = lb;
+      while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (B[mid] < x) lo = mid + 1;
+        else hi = mid;
+      }
+      hits += (lo < lb && B[lo] == x) ? 1u : 0u;
+    }
+    for (int off = 32; off > 0; off >>= 1) hits += __shfl_xor(hits, off);
+    if (lane == 0) out[t] = hits;
+  }
+}
+
+}  // namespace
+
+kcl_status add_edge_tasks(const kcl_graph *g, uint32_t k, const std::vector<uint32_t> &big,
+                          PassPlan *plan) {
+  plan->edge_tasks.clear();
+  for (uint32_t c = 0; c <= kNumClasses; ++c) plan->eclass_begin[c] = 0;
+  for (uint32_t c = 0; c < kNumClasses; ++c) plan->eclass_count[c] = 0;
+  if (big.empty()) return KCL_OK;
+  if (k < 3)
+    return fail(KCL_E_UNSUPPORTED, "node %u has out-degree %u > %u, supported for k >= 3 only",
+                big[0], g->cd[big[0] + 1] - g->cd[big[0]], kMaxRootDegree);
+  std::vector<Task> cand;
+  for (uint32_t s : big)
+    for (uint32_t j = g->cd[s]; j < g->cd[s + 1]; ++j) {
+      const uint32_t u = g->adj[j];
+      if (u & kEdgeTaskFlag) return fail(KCL_E_UNSUPPORTED, "node id %u >= 2^31 in an edge task", u);
+      cand.push_back(Task{s, kEdgeTaskFlag | u});
+    }
+  std::vector<uint32_t> members(cand.size());
+  {
+    DevBuf<Task> dt;
+    DevBuf<uint32_t> dm;
+    KCL_TRY(dt.alloc(cand.size()));
+    KCL_TRY(dm.alloc(cand.size()));
+    KCL_HIP(hipMemcpyAsync(dt.p, cand.data(), cand.size() * sizeof(Task), hipMemcpyHostToDevice,
+                           g->stream));
+    uint64_t blocks = (cand.size() + 3) / 4;
+    if (blocks > 65536) blocks = 65536;
+    edge_members_kernel<<<(uint32_t)blocks, 256, 0, g->stream>>>(g->d_cd.p, g->d_adj.p, dt.p,
+                                                                 (uint32_t)cand.size(), dm.p);
+    KCL_HIP(hipGetLastError());
+    KCL_HIP(hipMemcpyAsync(members.data(), dm.p, members.size() * 4, hipMemcpyDeviceToHost,
+                           g->stream));
+    KCL_HIP(hipStreamSynchronize(g->stream));
+  }
+  struct Entry {
+    uint32_t c;
+    Task t;
+  };
+  std::vector<Entry> cls[kNumClasses];
+  for (size_t i = 0; i < cand.size(); ++i) {
+    const uint32_t c = members[i];
+    if (c < k - 2) continue;  // no (k-2)-clique among the common members
+    if (c > kMaxRootDegree)
+      return fail(KCL_E_UNSUPPORTED,
+                  "edge (%u, %u) has %u common out-neighbours in the DAG; the enumerator "
+                  "handles up to %u",
+                  cand[i].s, task_edge_node(cand[i].part_np), c, kMaxRootDegree);
+    const uint32_t words = (c + 63) / 64;
+    uint32_t w = 0;
+    while (kClassWords[w] < words) ++w;
+    cls[w].push_back(Entry{c, cand[i]});
+  }
+  uint32_t at = 0;
+  for (uint32_t w = 0; w < kNumClasses; ++w) {
+    std::stable_sort(cls[w].begin(), cls[w].end(),
+                     [](const Entry &a, const Entry &b) { return a.c > b.c; });
+    plan->eclass_begin[w] = at;
+    for (const Entry &e : cls[w]) plan->edge_tasks.push_back(e.t);
+    plan->eclass_count[w] = (uint32_t)cls[w].size();
+    at += (uint32_t)cls[w].size();
+  }
+  plan->eclass_begin[kNumClasses] = at;
+  return KCL_OK;
+}
+
+kcl_status build_plan(const kcl_graph *g, uint32_t k, const RootCost *rc, uint32_t split_work,
+                      PassPlan *plan, const std::vector<uint64_t> *keep) {
+  const std::vector<uint32_t> &cd = g->cd;
   const uint32_t n = cd.empty() ? 0 : (uint32_t)(cd.size() - 1);
+  std::vector<uint32_t> big;
   struct Entry {
     uint64_t cost;
     uint32_t s;
@@ -106,10 +213,10 @@ kcl_status build_plan(const std::vector<uint32_t> &cd, uint32_t k, const RootCos
     const uint32_t d = cd[s + 1] - cd[s];
     if (d < need || d == 0) continue;
     if (keep && (*keep)[s] == 0) continue;
-    if (d > kMaxRootDegree)
-      return fail(KCL_E_UNSUPPORTED,
-                  "node %u has out-degree %u in the DAG; the enumerator handles up to %u", s, d,
-                  kMaxRootDegree);
+    if (d > g->edge_above) {
+      big.push_back(s);
+      continue;
+    }
     const uint32_t words = (d + 63) / 64;
     uint32_t c = 0;
     while (kClassWords[c] < words) ++c;  // smallest class that holds d members
@@ -145,41 +252,60 @@ kcl_status build_plan(const std::vector<uint32_t> &cd, uint32_t k, const RootCos
     at += cnt;
   }
   plan->class_begin[kNumClasses] = at;
-  return KCL_OK;
+  return add_edge_tasks(g, k, big, plan);
 }
+
+namespace {
+
+// Tasks of classes below min_class moved into min_class (the floor class's
+// own tasks first, then the promoted ones, widest first).
+void promote_group(std::vector<Task> &tasks, uint32_t *begin, uint32_t *count,
+                   uint32_t min_class) {
+  std::vector<Task> out;
+  out.reserve(tasks.size());
+  uint32_t nb[kNumClasses + 1] = {}, nc[kNumClasses] = {};
+  uint32_t at = 0;
+  for (uint32_t c = 0; c < kNumClasses; ++c) {
+    nb[c] = at;
+    if (c < min_class) continue;
+    for (uint32_t i = begin[c]; i < begin[c + 1]; ++i) out.push_back(tasks[i]);
+    if (c == min_class)
+      for (uint32_t l = min_class; l-- > 0;)
+        for (uint32_t i = begin[l]; i < begin[l + 1]; ++i) out.push_back(tasks[i]);
+    nc[c] = (uint32_t)out.size() - at;
+    at = (uint32_t)out.size();
+  }
+  nb[kNumClasses] = at;
+  tasks.swap(out);
+  for (uint32_t c = 0; c <= kNumClasses; ++c) begin[c] = nb[c];
+  for (uint32_t c = 0; c < kNumClasses; ++c) count[c] = nc[c];
+}
+
+}  // namespace
 
 void promote_plan(PassPlan *plan, uint32_t min_class) {
   if (min_class == 0 || min_class >= kNumClasses) return;
-  PassPlan out;
-  out.tasks.reserve(plan->tasks.size());
-  uint32_t at = 0;
-  for (uint32_t c = 0; c < kNumClasses; ++c) {
-    out.class_begin[c] = at;
-    if (c < min_class) continue;
-    // The floor class first (its own, heavier roots), then the promoted
-    // lighter classes, widest first.
-    for (uint32_t i = plan->class_begin[c]; i < plan->class_begin[c + 1]; ++i)
-      out.tasks.push_back(plan->tasks[i]);
-    if (c == min_class)
-      for (uint32_t l = min_class; l-- > 0;)
-        for (uint32_t i = plan->class_begin[l]; i < plan->class_begin[l + 1]; ++i)
-          out.tasks.push_back(plan->tasks[i]);
-    out.class_count[c] = (uint32_t)out.tasks.size() - at;
-    at = (uint32_t)out.tasks.size();
-  }
-  out.class_begin[kNumClasses] = at;
-  *plan = std::move(out);
+  promote_group(plan->tasks, plan->class_begin, plan->class_count, min_class);
+  promote_group(plan->edge_tasks, plan->eclass_begin, plan->eclass_count, min_class);
 }
 
 kcl_status upload_plan(const PassPlan &plan, DevicePlan *dev) {
   dev->host_tasks = plan.tasks;
-  KCL_TRY(dev->tasks.alloc(plan.tasks.size()));
-  KCL_TRY(dev->tickets.alloc(kNumClasses));
-  if (!plan.tasks.empty())
-    KCL_HIP(hipMemcpy(dev->tasks.p, plan.tasks.data(), plan.tasks.size() * sizeof(Task),
+  dev->host_tasks.insert(dev->host_tasks.end(), plan.edge_tasks.begin(), plan.edge_tasks.end());
+  const uint32_t nv = (uint32_t)plan.tasks.size();
+  KCL_TRY(dev->tasks.alloc(dev->host_tasks.size()));
+  KCL_TRY(dev->tickets.alloc(2 * kNumClasses));
+  if (!dev->host_tasks.empty())
+    KCL_HIP(hipMemcpy(dev->tasks.p, dev->host_tasks.data(), dev->host_tasks.size() * sizeof(Task),
                       hipMemcpyHostToDevice));
-  for (uint32_t c = 0; c <= kNumClasses; ++c) dev->class_begin[c] = plan.class_begin[c];
-  for (uint32_t c = 0; c < kNumClasses; ++c) dev->class_count[c] = plan.class_count[c];
+  for (uint32_t c = 0; c <= kNumClasses; ++c) {
+    dev->class_begin[c] = plan.class_begin[c];
+    dev->eclass_begin[c] = nv + plan.eclass_begin[c];
+  }
+  for (uint32_t c = 0; c < kNumClasses; ++c) {
+    dev->class_count[c] = plan.class_count[c];
+    dev->eclass_count[c] = plan.eclass_count[c];
+  }
   return KCL_OK;
 }
 

@@ -55,6 +55,8 @@ struct Lds {
   unsigned long long root_sum;  // the root's credits, summed over the waves
   uint32_t task;                // ticket broadcast
   uint32_t next_item;           // counting items handed out so far
+  uint32_t wcount[4];           // edge tasks: members found per wavefront
+  uint32_t ecount;              // edge tasks: members found so far
 };
 
 __device__ __forceinline__ uint32_t select_bit(unsigned long long x, uint32_t r) {
@@ -200,20 +202,84 @@ __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restric
     const unsigned long long t_start = prof ? __builtin_amdgcn_s_memtime() : 0ull;
     const Task tk = tasks[ti];
     const uint32_t s = tk.s;
-    const uint32_t part = tk.part_np & 0xffffu, np = tk.part_np >> 16;
-    const uint32_t b0 = cd[s];
-    const uint32_t d = cd[s + 1] - b0;
-    const uint64_t ks = key[s];
-
-    // Stage the members, their keys and adjacency extents.
-    for (uint32_t i = tid; i < d; i += NT) {
-      const uint32_t x = adj[b0 + i];
-      sh.node[i] = x;
-      sh.key[i] = key[x];
-      const uint32_t xb = cd[x];
-      sh.beg[i] = xb;
-      sh.pre[i + 1] = cd[x + 1] - xb;
-      sh.cred[i] = 0;
+    // A vertex task (root s, part of np) or an edge task (root edge (s, u),
+    // members N+(s) & N+(u), run by the (k-1) kernel: its cliques are the
+    // k-cliques whose two largest ids are s and u).  In an edge task the
+    // clique's winner among {s, u} is the one of smaller (key, id); members
+    // have smaller ids than both, so "member key <= that key" decides.
+    const bool edge_task = task_is_edge(tk.part_np);
+    const uint32_t part = edge_task ? 0u : (tk.part_np & 0xffffu);
+    const uint32_t np = edge_task ? 1u : (tk.part_np >> 16);
+    uint32_t d;
+    uint64_t ks;
+    uint32_t eu = s;  // edge tasks: the second root node u
+    if (!edge_task) {
+      const uint32_t b0 = cd[s];
+      d = cd[s + 1] - b0;
+      ks = key[s];
+      // Stage the members, their keys and adjacency extents.
+      for (uint32_t i = tid; i < d; i += NT) {
+        const uint32_t x = adj[b0 + i];
+        sh.node[i] = x;
+        sh.key[i] = key[x];
+        const uint32_t xb = cd[x];
+        sh.beg[i] = xb;
+        sh.pre[i + 1] = cd[x + 1] - xb;
+        sh.cred[i] = 0;
+      }
+    } else {
+      eu = task_edge_node(tk.part_np);
+      const uint64_t k_s = key[s], k_u = key[eu];
+      ks = (k_u <= k_s) ? k_u : k_s;
+      // Members: N+(s) & N+(u) in ascending order.  Each thread takes
+      // entries of the shorter list and searches the longer one; the hits
+      // are compacted in order (per-wavefront ballots, then wavefront
+      // offsets).  The plan sized the class by this intersection.
+      const uint32_t bs = cd[s], ls = cd[s + 1] - bs, bu = cd[eu], lu = cd[eu + 1] - bu;
+      const uint32_t *A = adj + (ls <= lu ? bs : bu);
+      const uint32_t *B = adj + (ls <= lu ? bu : bs);
+      const uint32_t la = ls <= lu ? ls : lu, lb = ls <= lu ? lu : ls;
+      const uint32_t wave = tid >> 6;
+      if (tid == 0) sh.ecount = 0;
+      __syncthreads();
+      for (uint32_t base = 0; base < la; base += NT) {
+        const uint32_t i = base + tid;
+        bool hit = false;
+        uint32_t x = 0;
+        if (i < la) {
+          x = A[i];
+          uint32_t lo = 0, hi = lb;
+          while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (B[mid] < x) lo = mid + 1;
+            else hi = mid;
+          }
+          hit = lo < lb && B[lo] == x;
+        }
+        const unsigned long long m = __ballot(hit);
+        if (lane == 0) sh.wcount[wave] = (uint32_t)__popcll(m);
+        __syncthreads();
+        uint32_t at = sh.ecount;
+        for (uint32_t w = 0; w < wave; ++w) at += sh.wcount[w];
+        at += (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+        if (hit && at < (uint32_t)Lds<W>::D) sh.node[at] = x;
+        __syncthreads();
+        if (tid == 0) {
+          uint32_t tot = 0;
+          for (uint32_t w = 0; w < NW; ++w) tot += sh.wcount[w];
+          sh.ecount += tot;
+        }
+        __syncthreads();
+      }
+      d = min(sh.ecount, (uint32_t)Lds<W>::D);
+      for (uint32_t i = tid; i < d; i += NT) {
+        const uint32_t x = sh.node[i];
+        sh.key[i] = key[x];
+        const uint32_t xb = cd[x];
+        sh.beg[i] = xb;
+        sh.pre[i + 1] = cd[x + 1] - xb;
+        sh.cred[i] = 0;
+      }
     }
     if (tid == 0) {
       sh.pre[0] = 0;
@@ -499,7 +565,14 @@ __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restric
     root_acc = wave_sum(root_acc);
     if (lane == 0 && root_acc) atomicAdd(&sh.root_sum, root_acc);
     __syncthreads();
-    if (tid == 0 && sh.root_sum) atomicAdd(&cnt[s], sh.root_sum);
+    if (tid == 0 && sh.root_sum) {
+      // The root's share: to s (root counts, and every clique's root in
+      // k-clique degrees); in an edge task to the winner of {s, u} and, for
+      // k-clique degrees, to u as well.
+      const uint32_t r = (edge_task && !ALL && !to_root && key[eu] <= ks) ? eu : s;
+      atomicAdd(&cnt[r], sh.root_sum);
+      if (ALL && edge_task) atomicAdd(&cnt[eu], sh.root_sum);
+    }
     if (prof && tid == 0) prof[2 * (size_t)ti + 1] = __builtin_amdgcn_s_memtime() - t_start;
     for (uint32_t i = tid; i < d; i += NT) {
       const unsigned long long c = sh.cred[i];

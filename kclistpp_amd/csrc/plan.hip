@@ -50,7 +50,7 @@ kcl_status root_counts(kcl_graph *g, uint32_t k, const PassPlan &plan, hipStream
 // parts amortise their rebuild).
 kcl_status root_costs(kcl_graph *g, uint32_t k, hipStream_t st, RootCost *out) {
   PassPlan p;
-  KCL_TRY(build_plan(g->cd, k - 1, nullptr, 0, &p));
+  KCL_TRY(build_plan(g, k - 1, nullptr, 0, &p));
   KCL_TRY(root_counts(g, k - 1, p, st, &out->work));
   out->build.assign(g->n, 0);
   for (uint32_t s = 0; s < g->n; ++s) {
@@ -80,10 +80,11 @@ constexpr double kLearnedDepth = 4.0;
 // a part of np costs B + C/np.  Within a class with S concurrent task
 // slots and total predicted time T, counting is split so that each part's
 // counting is at most max(T / (S kLearnedDepth), kBuildShare B).
-kcl_status build_plan_learned(const std::vector<uint32_t> &cd, uint32_t k,
-                              const std::vector<float> &pred,
+kcl_status build_plan_learned(const kcl_graph *g, uint32_t k, const std::vector<float> &pred,
                               const std::vector<uint64_t> *keep, PassPlan *plan) {
+  const std::vector<uint32_t> &cd = g->cd;
   const uint32_t n = cd.empty() ? 0 : (uint32_t)(cd.size() - 1);
+  std::vector<uint32_t> big;
   struct Entry {
     double b, c, cost;
     uint32_t s, np;
@@ -99,10 +100,10 @@ kcl_status build_plan_learned(const std::vector<uint32_t> &cd, uint32_t k,
     const uint32_t d = cd[s + 1] - cd[s];
     if (d < need || d == 0) continue;
     if (keep && (*keep)[s] == 0) continue;
-    if (d > kMaxRootDegree)
-      return fail(KCL_E_UNSUPPORTED,
-                  "node %u has out-degree %u in the DAG; the enumerator handles up to %u", s, d,
-                  kMaxRootDegree);
+    if (d > g->edge_above) {  // edge tasks
+      big.push_back(s);
+      continue;
+    }
     const uint32_t c = class_of(d);
     Entry e{cycles(pred[2 * (size_t)s]), cycles(pred[2 * (size_t)s + 1]), 0.0, s, 1};
     total[c] += e.b + e.c;
@@ -137,7 +138,7 @@ kcl_status build_plan_learned(const std::vector<uint32_t> &cd, uint32_t k,
     at += cnt;
   }
   plan->class_begin[kNumClasses] = at;
-  return KCL_OK;
+  return add_edge_tasks(g, k, big, plan);
 }
 
 }  // namespace
@@ -150,7 +151,7 @@ kcl_status root_features(kcl_graph *g, uint32_t k, hipStream_t st, std::vector<f
   std::vector<uint64_t> tri(n, 0);
   if (g->core >= 2) {
     PassPlan p3;
-    KCL_TRY(build_plan(g->cd, 3, nullptr, 1u << 17, &p3));
+    KCL_TRY(build_plan(g, 3, nullptr, 1u << 17, &p3));
     KCL_TRY(root_counts(g, 3, p3, st, &tri));
   }
   const std::vector<uint32_t> &cd = g->cd;
@@ -208,14 +209,14 @@ kcl_status plan_for_schedule(kcl_graph *g, uint32_t k, uint32_t schedule,
   if (schedule == KCL_SCHED_LEARNED && model) {
     std::vector<float> pred;
     KCL_TRY(learned_predictions(g, k, model, st, &pred));
-    return build_plan_learned(g->cd, k, pred, keep, out);
+    return build_plan_learned(g, k, pred, keep, out);
   }
   if (schedule == KCL_SCHED_COUNTED && k >= 4) {
     RootCost cost;
     KCL_TRY(root_costs(g, k, st, &cost));
-    return build_plan(g->cd, k, &cost, split, out, keep);
+    return build_plan(g, k, &cost, split, out, keep);
   }
-  return build_plan(g->cd, k, nullptr, split, out, keep);
+  return build_plan(g, k, nullptr, split, out, keep);
 }
 
 kcl_status make_plan(kcl_graph *g, uint32_t k, uint32_t split, hipStream_t st, PassPlan *out,
@@ -231,7 +232,7 @@ kcl_status make_plan(kcl_graph *g, uint32_t k, uint32_t split, hipStream_t st, P
   std::vector<float> pred;
   if (learned) {
     KCL_TRY(learned_predictions(g, k, model, st, &pred));
-    KCL_TRY(build_plan_learned(g->cd, k, pred, candidates, &all));
+    KCL_TRY(build_plan_learned(g, k, pred, candidates, &all));
   } else {
     KCL_TRY(plan_for_schedule(g, k, schedule, model, split, candidates, st, &all));
   }
@@ -239,7 +240,7 @@ kcl_status make_plan(kcl_graph *g, uint32_t k, uint32_t split, hipStream_t st, P
   KCL_TRY(root_counts(g, k, all, st, &ck));
   if (learned) {
     // Same predictions, kept roots only (the class totals change).
-    KCL_TRY(build_plan_learned(g->cd, k, pred, &ck, out));
+    KCL_TRY(build_plan_learned(g, k, pred, &ck, out));
   } else {
     // Reuse the candidate plan's costs: drop roots without k-cliques.
     PassPlan kept;
@@ -253,6 +254,15 @@ kcl_status make_plan(kcl_graph *g, uint32_t k, uint32_t split, hipStream_t st, P
       at = (uint32_t)kept.tasks.size();
     }
     kept.class_begin[kNumClasses] = at;
+    at = 0;
+    for (uint32_t c = 0; c < kNumClasses; ++c) {
+      kept.eclass_begin[c] = at;
+      for (uint32_t i = all.eclass_begin[c]; i < all.eclass_begin[c + 1]; ++i)
+        if (ck[all.edge_tasks[i].s]) kept.edge_tasks.push_back(all.edge_tasks[i]);
+      kept.eclass_count[c] = (uint32_t)kept.edge_tasks.size() - at;
+      at = (uint32_t)kept.edge_tasks.size();
+    }
+    kept.eclass_begin[kNumClasses] = at;
     *out = std::move(kept);
   }
   if (info) {
@@ -266,7 +276,7 @@ kcl_status make_plan(kcl_graph *g, uint32_t k, uint32_t split, hipStream_t st, P
     }
     info->roots_total = roots_all;
     info->roots_kept = kept;
-    info->tasks = (uint32_t)out->tasks.size();
+    info->tasks = (uint32_t)(out->tasks.size() + out->edge_tasks.size());
     info->cliques = cliques;
     info->plan_ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();

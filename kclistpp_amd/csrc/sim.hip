@@ -473,8 +473,9 @@ extern "C" kcl_status kcl_sim_profile_pass(kcl_sim *s, uint32_t *ntasks, uint32_
   for (uint32_t i = 0; i < nt; ++i) {
     const kcl::Task &t = s->plan.host_tasks[i];
     if (root) root[i] = t.s;
-    if (part) part[i] = t.part_np & 0xffffu;
-    if (nparts) nparts[i] = t.part_np >> 16;
+    const bool et = kcl::task_is_edge(t.part_np);  // edge task: part 0 of 0
+    if (part) part[i] = et ? 0u : (t.part_np & 0xffffu);
+    if (nparts) nparts[i] = et ? 0u : (t.part_np >> 16);
     if (build_cycles) build_cycles[i] = h[2 * (size_t)i];
     if (total_cycles) total_cycles[i] = h[2 * (size_t)i + 1];
   }
@@ -557,7 +558,7 @@ extern "C" kcl_status kcl_sim_replan(kcl_sim *s, uint32_t schedule, const kcl_co
                                  &p));
   KCL_TRY(kcl::upload_plan(p, &s->plan));
   KCL_HIP(hipStreamSynchronize(s->stream));
-  s->info.tasks = (uint32_t)p.tasks.size();
+  s->info.tasks = (uint32_t)(p.tasks.size() + p.edge_tasks.size());
   s->info.plan_ms =
       std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return KCL_OK;

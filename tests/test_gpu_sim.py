@@ -224,3 +224,74 @@ def test_trivial_and_empty_graphs(kcl):
     with pytest.raises(kcl.KclError) as e:
         kcl.Graph(np.array([[3, 3]], np.uint32))
     assert e.value.name == "KCL_E_INVALID_ARG"
+
+
+@pytest.mark.parametrize("above,k", [(8, 3), (8, 4), (8, 5), (12, 6), (12, 7), (30, 5), (30, 8)])
+def test_edge_tasks_vs_oracle(kcl, oracle, above, k):
+    """Roots above the threshold run as edge tasks (s, u) over
+    N+(s) & N+(u) with the (k-1) kernels; counts must not change."""
+    pairs = _community_graph(300 + k, size=50, p=0.8)
+    og = oracle.dag(pairs)
+    rng = np.random.default_rng(k)
+    rho = rng.integers(0, 4, og.n).astype(np.float64) * 0.25
+    level = rng.permutation(og.n).astype(np.uint32)
+    _say("edge tasks above", above, "k", k, "core", og.core)
+    assert og.core > above
+    with kcl.Graph(pairs) as g:
+        g.set_edge_split(above)
+        acc, visits = g.clique_enum(k, kcl.FRANK_WOLFE, rho=rho)
+        want, wv = oracle.fw_counts(og, k, rho)
+        assert visits == wv and np.array_equal(acc, want.astype(np.float64))
+        acc, _ = g.clique_enum(k, kcl.PAVA_PREPROCESS, level=level)
+        assert np.array_equal(acc, oracle.pava_counts(og, k, level).astype(np.float64))
+
+
+def test_edge_tasks_in_sim_trim_and_exact(kcl):
+    pairs = _community_graph(41, size=45, p=0.85)
+    with kcl.Graph(pairs) as g, kcl.Graph(pairs) as ge:
+        ge.set_edge_split(10)
+        for k in (4, 6):
+            a, b = kcl.Sim(g, k), kcl.Sim(ge, k)
+            assert b.plan_info()["tasks"] > a.plan_info()["tasks"]
+            for t in (1, 2, 3, 4):
+                a.step(t)
+                b.step(t)
+                assert np.array_equal(a.counts(), b.counts())
+                assert np.array_equal(a.rho().view(np.uint64), b.rho().view(np.uint64))
+            assert a.find_densest() == b.find_densest()
+            # cost samples aggregate edge tasks per root
+            roots, x, y = b.cost_samples()
+            assert len(roots) == b.plan_info()["roots_kept"] and np.all(np.isfinite(y))
+            a.close()
+            b.close()
+        ra, na = kcl.exact(g, 4)
+        rb, nb = kcl.exact(ge, 4)
+        assert ra["certified"] == rb["certified"] == 1
+        assert (ra["n"], ra["cliques"], ra["m"]) == (rb["n"], rb["cliques"], rb["m"])
+        assert np.array_equal(na, nb)
+
+
+def test_degeneracy_above_1024(kcl, oracle):
+    """Complete tripartite K_{520,520,520}: every node has DAG out-degree
+    up to 1040 > 1024, common out-neighbourhoods stay within one part."""
+    a = 520
+    parts = [np.arange(i * a, (i + 1) * a) for i in range(3)]
+    pairs = np.concatenate([
+        np.stack(np.meshgrid(parts[i], parts[j], indexing="ij"), -1).reshape(-1, 2)
+        for i, j in ((0, 1), (0, 2), (1, 2))]).astype(np.uint32)
+    rng = np.random.default_rng(3)
+    pairs = gg.finalize(pairs, rng)
+    og = oracle.dag(pairs)
+    assert og.core > 1024
+    rho = rng.integers(0, 3, og.n).astype(np.float64)
+    _say("tripartite core", og.core)
+    with kcl.Graph(pairs) as g:
+        acc, visits = g.clique_enum(3, kcl.FRANK_WOLFE, rho=rho)
+        assert visits == a ** 3
+        want, wv = oracle.fw_counts(og, 3, rho)
+        assert wv == a ** 3 and np.array_equal(acc, want.astype(np.float64))
+        acc4, v4 = g.clique_enum(4, kcl.FRANK_WOLFE, rho=rho)
+        assert v4 == 0 and not acc4.any()
+        with kcl.Sim(g, 3) as s:
+            s.step()
+            assert s.counts().sum() == a ** 3
