@@ -203,11 +203,12 @@ kcl_status build_plan(const kcl_graph *g, uint32_t k, const RootCost *rc, uint32
   const uint32_t n = cd.empty() ? 0 : (uint32_t)(cd.size() - 1);
   std::vector<uint32_t> big;
   struct Entry {
-    uint64_t cost;
+    uint64_t cost, work, build;
     uint32_t s;
     uint32_t np;
   };
   std::vector<Entry> cls[kNumClasses];
+  double total[kNumClasses] = {};
   const uint32_t need = k - 1;  // members needed in N+(s)
   for (uint32_t s = 0; s < n; ++s) {
     const uint32_t d = cd[s + 1] - cd[s];
@@ -222,19 +223,29 @@ kcl_status build_plan(const kcl_graph *g, uint32_t k, const RootCost *rc, uint32
     while (kClassWords[c] < words) ++c;  // smallest class that holds d members
     const uint64_t work = rc ? rc->work[s] : (uint64_t)d * d;
     const uint64_t build = rc ? rc->build[s] : (uint64_t)d;
-    const uint64_t cost = work + build;
-    uint32_t np = 1;
-    // Each part rebuilds the root's rows, so a part takes at least
-    // kBuildShare times the build's work units of counting.
-    const uint64_t part_work = std::max<uint64_t>(split_work, kBuildShare * build);
-    if (split_work && work > part_work) {
-      uint64_t want = (work + part_work - 1) / part_work;
-      const uint64_t items = (k >= 5) ? (uint64_t)d * (d - 1) / 2 : d;
-      if (want > items) want = items;
-      if (want > 4096) want = 4096;
-      np = (uint32_t)(want ? want : 1);
+    cls[c].push_back(Entry{work + build, work, build, s, 1});
+    total[c] += (double)(work + build);
+  }
+  for (uint32_t c = 0; c < kNumClasses; ++c) {
+    // Counted costs: a part carries at most a quarter of one resident
+    // workgroup's fair share of its class (and at least split_work), so
+    // neither a heavy root nor the rebuilds of many parts set the class's
+    // time.  The plain d+^2 schedule keeps the absolute split_work.
+    const double tau = rc ? total[c] / (4.0 * std::max<uint32_t>(1, class_slots(c))) : 0.0;
+    for (Entry &e : cls[c]) {
+      const uint32_t d = cd[e.s + 1] - cd[e.s];
+      // Each part rebuilds the root's rows, so a part takes at least
+      // kBuildShare times the build's work units of counting.
+      const uint64_t part_work =
+          std::max<uint64_t>(std::max<uint64_t>(split_work, kBuildShare * e.build), (uint64_t)tau);
+      if (split_work && e.work > part_work) {
+        uint64_t want = (e.work + part_work - 1) / part_work;
+        const uint64_t items = (k >= 5) ? (uint64_t)d * (d - 1) / 2 : d;
+        if (want > items) want = items;
+        if (want > 4096) want = 4096;
+        e.np = (uint32_t)(want ? want : 1);
+      }
     }
-    cls[c].push_back(Entry{cost, s, np});
   }
   plan->tasks.clear();
   uint32_t at = 0;
