@@ -63,16 +63,16 @@ def algorithmic_bytes(cd: np.ndarray, adj: np.ndarray, roots: np.ndarray) -> int
     return 16 * int(roots.sum()) + int(per_edge[roots[src]].sum()) + 8 * n
 
 
-def profiled_traffic(workload: str):
-    """HBM bytes per pass from the committed rocprofv3 summary of this
-    workload (FETCH_SIZE and WRITE_SIZE, scaled by the copy-kernel factors),
-    or None when there is none."""
+def profiled(workload: str, key: str):
+    """A figure from the committed rocprofv3 summary of this workload
+    (traffic_bytes_per_pass: FETCH_SIZE + WRITE_SIZE scaled by the
+    copy-kernel factors; lds_frac_upper_per_pass), or None."""
     path = os.path.join(ROOT, "profiles", f"pass_kernel_{workload}.md")
     if not os.path.exists(path):
         return None
     for line in open(path):
-        if line.startswith("traffic_bytes_per_pass:"):
-            return int(line.split(":")[1].split()[0])
+        if line.startswith(key + ":"):
+            return float(line.split(":")[1].split()[0])
     return None
 
 
@@ -375,7 +375,9 @@ def main():
     achieved = alg_bytes / (kern_avg_ms * 1e-3) / 1e9 if kern_avg_ms > 0 else 0.0
     roofline = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6),
-                "traffic": profiled_traffic(args.workload),
+                "traffic": (lambda v: int(v) if v is not None else None)(
+                    profiled(args.workload, "traffic_bytes_per_pass")),
+                "lds_frac_upper": profiled(args.workload, "lds_frac_upper_per_pass"),
                 "kernel": "pass_kernel (all word classes of one enumeration pass)",
                 "kernel_ms": round(kern_avg_ms, 4), "algorithmic_bytes": alg_bytes}
     cpu = None
