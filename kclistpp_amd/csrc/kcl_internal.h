@@ -100,6 +100,9 @@ struct PassPlan {
   std::vector<Task> edge_tasks;
   uint32_t eclass_begin[kNumClasses + 1] = {};
   uint32_t eclass_count[kNumClasses] = {};
+  // Per class, the index (within the class) from which tasks are light
+  // enough to be claimed in chunks; class_count means none.
+  uint32_t light_from[kNumClasses] = {};
 };
 
 struct DevicePlan {
@@ -110,6 +113,7 @@ struct DevicePlan {
   uint32_t class_count[kNumClasses] = {};
   uint32_t eclass_begin[kNumClasses + 1] = {};  // offsets into tasks
   uint32_t eclass_count[kNumClasses] = {};
+  uint32_t light_from[kNumClasses] = {};
   uint32_t max_per_cu = 0;  // cap on resident workgroups per CU (0: by LDS)
 };
 
@@ -152,6 +156,10 @@ kcl_status build_plan(const kcl_graph *g, uint32_t k, const RootCost *rc, uint32
 kcl_status add_edge_tasks(const kcl_graph *g, uint32_t k, const std::vector<uint32_t> &big,
                           PassPlan *plan);
 kcl_status upload_plan(const PassPlan &plan, DevicePlan *dev);
+// Tasks cheaper than a resident workgroup's share of its class divided by
+// kLightShare are claimed in chunks (light_cost gives that bound).
+constexpr double kLightShare = 64.0;
+double light_cost(double class_total, uint32_t wclass);
 // Moves the tasks of classes below min_class into min_class (wider rows:
 // a class-c kernel handles any root with d+ <= 64 kClassWords[c]).
 void promote_plan(PassPlan *plan, uint32_t min_class);

@@ -51,8 +51,11 @@ uint32_t blocks_per_cu(uint32_t wclass, const DeviceProps &props) {
 
 namespace {
 
+// Two counters per class (single claims, chunked claims), vertex and edge.
+constexpr uint32_t kTickets = 2 * 2 * kNumClasses;
+
 __global__ void zero_tickets_kernel(uint32_t *t) {
-  if (threadIdx.x < 2 * kNumClasses) t[threadIdx.x] = 0;
+  if (threadIdx.x < kTickets) t[threadIdx.x] = 0;
 }
 
 }  // namespace
@@ -69,7 +72,7 @@ kcl_status launch_pass(const kcl_graph *g, const uint64_t *d_key, uint32_t k, Pa
   DeviceProps props;
   KCL_TRY(device_props(&props));
   // A kernel rather than a memset node, so the pass can be graph-captured.
-  static_assert(2 * kNumClasses <= 64, "tickets of one pass fit one wavefront");
+  static_assert(kTickets <= 64, "tickets of one pass fit one wavefront");
   zero_tickets_kernel<<<1, 64, 0, stream>>>(plan.tickets.p);
   KCL_HIP(hipGetLastError());
   // Vertex tasks with the k kernels, then edge tasks with the (k-1) ones.
@@ -91,7 +94,8 @@ kcl_status launch_pass(const kcl_graph *g, const uint64_t *d_key, uint32_t k, Pa
                          (const uint32_t *)g->d_cd.p, (const uint32_t *)g->d_adj.p, d_key,
                          (const unsigned long long *)(g->core_h ? g->d_core.p : nullptr),
                          g->core_h, plan.tasks.p + begin[c], count,
-                         plan.tickets.p + grp * kNumClasses + c, (uint32_t)mode, d_cnt,
+                         grp ? count : std::min(plan.light_from[c], count),
+                         plan.tickets.p + 2 * (grp * kNumClasses + c), (uint32_t)mode, d_cnt,
                          prof ? prof + 2 * (size_t)begin[c] : nullptr);
       KCL_HIP(hipGetLastError());
     }
@@ -197,6 +201,10 @@ kcl_status add_edge_tasks(const kcl_graph *g, uint32_t k, const std::vector<uint
   return KCL_OK;
 }
 
+double light_cost(double class_total, uint32_t wclass) {
+  return class_total / ((double)std::max<uint32_t>(1, class_slots(wclass)) * kLightShare);
+}
+
 kcl_status build_plan(const kcl_graph *g, uint32_t k, const RootCost *rc, uint32_t split_work,
                       PassPlan *plan, const std::vector<uint64_t> *keep) {
   const std::vector<uint32_t> &cd = g->cd;
@@ -253,13 +261,18 @@ kcl_status build_plan(const kcl_graph *g, uint32_t k, const RootCost *rc, uint32
     std::stable_sort(cls[c].begin(), cls[c].end(),
                      [](const Entry &a, const Entry &b) { return a.cost > b.cost; });
     plan->class_begin[c] = at;
+    const double light = light_cost(total[c], c);
     uint32_t cnt = 0;
+    plan->light_from[c] = UINT32_MAX;
     for (const Entry &e : cls[c])
       for (uint32_t p = 0; p < e.np; ++p) {
+        if (plan->light_from[c] == UINT32_MAX && (double)e.cost / e.np < light)
+          plan->light_from[c] = cnt;
         plan->tasks.push_back(Task{e.s, p | (e.np << 16)});
         ++cnt;
       }
     plan->class_count[c] = cnt;
+    plan->light_from[c] = std::min(plan->light_from[c], cnt);
     at += cnt;
   }
   plan->class_begin[kNumClasses] = at;
@@ -296,6 +309,9 @@ void promote_group(std::vector<Task> &tasks, uint32_t *begin, uint32_t *count,
 
 void promote_plan(PassPlan *plan, uint32_t min_class) {
   if (min_class == 0 || min_class >= kNumClasses) return;
+  // The promoted (lighter) tasks follow the floor class's own: they stay
+  // in its light tail.
+  for (uint32_t c = 0; c < min_class; ++c) plan->light_from[c] = 0;
   promote_group(plan->tasks, plan->class_begin, plan->class_count, min_class);
   promote_group(plan->edge_tasks, plan->eclass_begin, plan->eclass_count, min_class);
 }
@@ -305,7 +321,7 @@ kcl_status upload_plan(const PassPlan &plan, DevicePlan *dev) {
   dev->host_tasks.insert(dev->host_tasks.end(), plan.edge_tasks.begin(), plan.edge_tasks.end());
   const uint32_t nv = (uint32_t)plan.tasks.size();
   KCL_TRY(dev->tasks.alloc(dev->host_tasks.size()));
-  KCL_TRY(dev->tickets.alloc(2 * kNumClasses));
+  KCL_TRY(dev->tickets.alloc(kTickets));
   if (!dev->host_tasks.empty())
     KCL_HIP(hipMemcpy(dev->tasks.p, dev->host_tasks.data(), dev->host_tasks.size() * sizeof(Task),
                       hipMemcpyHostToDevice));
@@ -316,6 +332,7 @@ kcl_status upload_plan(const PassPlan &plan, DevicePlan *dev) {
   for (uint32_t c = 0; c < kNumClasses; ++c) {
     dev->class_count[c] = plan.class_count[c];
     dev->eclass_count[c] = plan.eclass_count[c];
+    dev->light_from[c] = plan.light_from[c];
   }
   return KCL_OK;
 }

@@ -57,6 +57,7 @@ struct Lds {
   uint32_t next_item;           // counting items handed out so far
   uint32_t wcount[4];           // edge tasks: members found per wavefront
   uint32_t ecount;              // edge tasks: members found so far
+  uint32_t chunk;               // 0 while claiming single tasks, else the chunk size
 };
 
 __device__ __forceinline__ uint32_t select_bit(unsigned long long x, uint32_t r) {
@@ -180,7 +181,7 @@ __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restric
                                                        const unsigned long long *__restrict__ core,
                                                        uint32_t core_h,
                                                        const Task *__restrict__ tasks,
-                                                       uint32_t ntasks,
+                                                       uint32_t ntasks, uint32_t light_from,
                                                        uint32_t *__restrict__ ticket,
                                                        uint32_t mode,
                                                        unsigned long long *__restrict__ cnt,
@@ -192,10 +193,35 @@ __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restric
   const uint32_t lane = tid & 63;
   const bool to_root = !ALL && (mode == kCreditRoot);
 
+  // Tickets: tasks [0, light_from) one at a time from ticket[0] (heavy
+  // first), then the light tail [light_from, ntasks) in chunks of
+  // kLightChunk from ticket[1] (one same-address atomic per chunk).
+  constexpr uint32_t kLightChunk = 4;
+  uint32_t chunk_base = 0, chunk_left = 0, chunk_size = 1;
+  if (tid == 0) sh.chunk = 0;
+  __syncthreads();
   for (;;) {
-    if (tid == 0) sh.task = atomicAdd(ticket, 1u);
-    __syncthreads();
-    const uint32_t ti = sh.task;
+    if (chunk_left == 0) {
+      if (tid == 0) {
+        uint32_t t = light_from, c = 0;
+        if (sh.chunk == 0) {  // still in the heavy part
+          t = atomicAdd(&ticket[0], 1u);
+          c = 1;
+        }
+        if (t >= light_from) {
+          t = light_from + atomicAdd(&ticket[1], kLightChunk);
+          c = kLightChunk;
+        }
+        sh.task = t;
+        sh.chunk = c == 1 ? 0u : c;
+      }
+      __syncthreads();
+      chunk_base = sh.task;
+      chunk_size = sh.chunk == 0 ? 1u : sh.chunk;
+      chunk_left = chunk_size;
+    }
+    const uint32_t ti = chunk_base + (chunk_size - chunk_left);
+    --chunk_left;
     if (ti >= ntasks) break;  // same value in every thread of the workgroup
     // Optional per-task timing (shader clock): prof[2 ti] = cycles until the
     // rows are built, prof[2 ti + 1] = cycles of the whole task.
@@ -585,8 +611,8 @@ __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restric
 }  // namespace
 
 using PassFn = void (*)(const uint32_t *, const uint32_t *, const uint64_t *,
-                        const unsigned long long *, uint32_t, const Task *, uint32_t, uint32_t *,
-                        uint32_t, unsigned long long *, unsigned long long *);
+                        const unsigned long long *, uint32_t, const Task *, uint32_t, uint32_t,
+                        uint32_t *, uint32_t, unsigned long long *, unsigned long long *);
 
 constexpr int kMaxTemplK = 10;  // k handled by compiled kernels: 2..kMaxTemplK
 

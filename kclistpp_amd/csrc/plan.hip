@@ -128,13 +128,17 @@ kcl_status build_plan_learned(const kcl_graph *g, uint32_t k, const std::vector<
     std::stable_sort(cls[c].begin(), cls[c].end(),
                      [](const Entry &a, const Entry &b) { return a.cost > b.cost; });
     plan->class_begin[c] = at;
+    const double light = light_cost(total[c], c);
     uint32_t cnt = 0;
+    plan->light_from[c] = UINT32_MAX;
     for (const Entry &e : cls[c])
       for (uint32_t p = 0; p < e.np; ++p) {
+        if (plan->light_from[c] == UINT32_MAX && e.cost < light) plan->light_from[c] = cnt;
         plan->tasks.push_back(Task{e.s, p | (e.np << 16)});
         ++cnt;
       }
     plan->class_count[c] = cnt;
+    plan->light_from[c] = std::min(plan->light_from[c], cnt);
     at += cnt;
   }
   plan->class_begin[kNumClasses] = at;
@@ -248,9 +252,15 @@ kcl_status make_plan(kcl_graph *g, uint32_t k, uint32_t split, hipStream_t st, P
     uint32_t at = 0;
     for (uint32_t c = 0; c < kNumClasses; ++c) {
       kept.class_begin[c] = at;
-      for (uint32_t i = all.class_begin[c]; i < all.class_begin[c + 1]; ++i)
-        if (ck[all.tasks[i].s]) kept.tasks.push_back(all.tasks[i]);
+      kept.light_from[c] = UINT32_MAX;
+      for (uint32_t i = all.class_begin[c]; i < all.class_begin[c + 1]; ++i) {
+        if (!ck[all.tasks[i].s]) continue;
+        if (kept.light_from[c] == UINT32_MAX && i - all.class_begin[c] >= all.light_from[c])
+          kept.light_from[c] = (uint32_t)kept.tasks.size() - at;
+        kept.tasks.push_back(all.tasks[i]);
+      }
       kept.class_count[c] = (uint32_t)kept.tasks.size() - at;
+      kept.light_from[c] = std::min(kept.light_from[c], kept.class_count[c]);
       at = (uint32_t)kept.tasks.size();
     }
     kept.class_begin[kNumClasses] = at;
