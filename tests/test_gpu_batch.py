@@ -119,8 +119,11 @@ def test_batch_counts_match_oracle_per_iteration(kcl, oracle, small):
 def test_configurations_do_not_change_results(kcl, small):
     _, graphs, ks = small
     res = []
+    import os
+    model = kcl.CostModel(path=os.path.join(kcl.PKG, "cost_model.kclcm"))
     for opts in ({}, {"tune": True}, {"row_bits": 256, "per_cu": 8},
-                 {"schedule": kcl.SCHED_DEGREE, "row_bits": 128}):
+                 {"schedule": kcl.SCHED_DEGREE, "row_bits": 128},
+                 {"schedule": kcl.SCHED_LEARNED, "model": model}):
         with kcl.Batch(graphs, ks, **opts) as b:
             for _ in range(3):
                 b.step()
@@ -129,6 +132,7 @@ def test_configurations_do_not_change_results(kcl, small):
             res.append([b.counts(i) for i in range(len(graphs))])
             if opts.get("row_bits"):
                 assert info["row_bits"] == opts["row_bits"]
+    model.close()
     for r in res[1:]:
         for a, b_ in zip(res[0], r):
             assert np.array_equal(a, b_)
