@@ -144,7 +144,8 @@ typedef struct kcl_cost_model kcl_cost_model;
 typedef struct kcl_sim_opts {
   uint32_t schedule;      /* kcl_schedule */
   uint32_t split_work;    /* COUNTED/DEGREE: work units per split part; 0 = default */
-  const kcl_cost_model *model; /* LEARNED: the model (nullable) */
+  const kcl_cost_model *model; /* LEARNED: the model (nullable; read only while
+                                  the plan is built, in kcl_sim_create) */
   uint32_t reserved[4];
 } kcl_sim_opts;
 
