@@ -223,6 +223,14 @@ def run_batch(args, kcl, gg):
         kern_ms += ms
     elapsed = time.perf_counter() - t_start
     log(f"bench batch: {args.steps} steps in {elapsed:.3f}s")
+    # The C4 run itself: CDF_Main with T = 50 for every graph (32 iterations,
+    # reports at T = 1, 2, 4, ..., 32), from rho = 0.
+    b.reset()
+    r0 = time.perf_counter()
+    recs = b.run(50)
+    run_s = time.perf_counter() - r0
+    log(f"bench batch: run(50) of all graphs in {run_s:.2f}s, "
+        f"{sum(len(r) for r in recs)} records")
     cpu = None if args.no_cpu_baseline else cpu_baseline_batch(pairs_list, ks)
     b.close()
     if model is not None:
@@ -247,6 +255,7 @@ def run_batch(args, kcl, gg):
                    "row_bits": info["row_bits"], "per_cu": info["per_cu"],
                    "tasks": info["tasks"], "plan_ms": info["plan_ms"],
                    "tune_ms": info["tune_ms"], "gpu_ms_per_step": kern_ms / max(args.steps, 1),
+                   "run50_s": run_s,
                    "parallelism": "1 GPU"},
         "roofline": None,
         "cpu_baseline": cpu,

@@ -332,6 +332,8 @@ kcl_status kcl_batch_get_info(const kcl_batch *b, kcl_batch_info *info);
 kcl_status kcl_batch_graph_cliques(const kcl_batch *b, uint64_t *out);
 /* Iteration t >= 1 of every graph (one hipGraph replay). */
 kcl_status kcl_batch_step(kcl_batch *b, uint32_t t);
+/* Back to the state after create: rho, counts and totals zero. */
+kcl_status kcl_batch_reset(kcl_batch *b);
 /* Visits of the last iteration (all graphs) and its GPU time. */
 kcl_status kcl_batch_last_pass(const kcl_batch *b, uint64_t *visits, double *ms);
 /* Per graph i (n_i entries): last counts, rho, exact totals V. */
@@ -340,8 +342,10 @@ kcl_status kcl_batch_rho(kcl_batch *b, uint32_t i, double *out);
 kcl_status kcl_batch_totals(kcl_batch *b, uint32_t i, uint64_t *out);
 /* CDF_FindDensestSubset of every graph (count entries). */
 kcl_status kcl_batch_find_densest(kcl_batch *b, kcl_densest_info *out);
-/* CDF_Main of every graph: iterations up to the largest power of two <=
- * num_iter, a record per power of two; records: count x 32, nrec: count. */
+/* CDF_Main of every graph: iterations 1 .. the largest power of two <=
+ * num_iter from the current state (a fresh or reset batch gives the
+ * reference's run), a record per power of two; records: count x 32,
+ * nrec: count. */
 kcl_status kcl_batch_run(kcl_batch *b, uint32_t num_iter, kcl_record *records, uint32_t *nrec);
 
 /* ---------------------------------------------------------------------

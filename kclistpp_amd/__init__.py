@@ -136,7 +136,8 @@ EXPORTS = [
     "kcl_cost_model_save", "kcl_cost_model_get", "kcl_cost_model_set", "kcl_cost_model_predict",
     "kcl_cost_model_gradient", "kcl_cost_model_train", "kcl_sim_root_features",
     "kcl_sim_cost_samples", "kcl_sim_replan", "kcl_batch_create", "kcl_batch_destroy",
-    "kcl_batch_get_info", "kcl_batch_graph_cliques", "kcl_batch_step", "kcl_batch_last_pass",
+    "kcl_batch_get_info", "kcl_batch_graph_cliques", "kcl_batch_step", "kcl_batch_reset",
+    "kcl_batch_last_pass",
     "kcl_batch_counts", "kcl_batch_rho", "kcl_batch_totals", "kcl_batch_find_densest",
     "kcl_batch_run",
 ]
@@ -210,6 +211,7 @@ def lib() -> C.CDLL:
         "kcl_batch_get_info": ([vp, C.POINTER(_BatchInfo)], C.c_int),
         "kcl_batch_graph_cliques": ([vp, vp], C.c_int),
         "kcl_batch_step": ([vp, u32], C.c_int),
+        "kcl_batch_reset": ([vp], C.c_int),
         "kcl_batch_last_pass": ([vp, C.POINTER(u64), C.POINTER(C.c_double)], C.c_int),
         "kcl_batch_counts": ([vp, u32, vp], C.c_int),
         "kcl_batch_rho": ([vp, u32, vp], C.c_int),
@@ -649,6 +651,11 @@ class Batch:
         t = self.t + 1 if t is None else t
         _check(lib().kcl_batch_step(self._h, t))
         self.t = t
+
+    def reset(self) -> None:
+        """rho, counts and totals back to zero (the state after create)."""
+        _check(lib().kcl_batch_reset(self._h))
+        self.t = 0
 
     def last_pass(self) -> tuple[int, float]:
         v = C.c_uint64(0)

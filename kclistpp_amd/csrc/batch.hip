@@ -546,6 +546,20 @@ extern "C" kcl_status kcl_batch_graph_cliques(const kcl_batch *b, uint64_t *out)
   return KCL_OK;
 }
 
+extern "C" kcl_status kcl_batch_reset(kcl_batch *b) {
+  if (!b) return fail(KCL_E_INVALID_ARG, "NULL argument");
+  const uint64_t n = b->nodes();
+  hipStream_t st = b->u->stream;
+  KCL_HIP(hipMemsetAsync(b->rho.p, 0, (size_t)(n ? n : 1) * sizeof(double), st));
+  KCL_TRY(kcl::zero_u64(b->cnt.p, n, st));
+  KCL_TRY(kcl::zero_u64(b->last.p, n, st));
+  KCL_TRY(kcl::zero_u64(b->V.p, n, st));
+  KCL_HIP(hipStreamSynchronize(st));
+  b->last_visits = 0;
+  b->last_ms = 0;
+  return KCL_OK;
+}
+
 extern "C" kcl_status kcl_batch_step(kcl_batch *b, uint32_t t) {
   if (!b || t == 0) return fail(KCL_E_INVALID_ARG, "NULL batch or t = 0");
   return kcl::batch_step(b, t);

@@ -86,6 +86,9 @@ def test_batch_run_matches_oracle(kcl, oracle, small):
     pairs, graphs, ks = small
     with kcl.Batch(graphs, ks) as b:
         cl = b.graph_cliques()
+        for _ in range(3):  # state that reset() must clear
+            b.step()
+        b.reset()
         recs = b.run(10)
     for i, (p, k) in enumerate(zip(pairs, ks)):
         if cl[i] > 300_000:  # keep the single-threaded oracle quick
