@@ -52,7 +52,7 @@ def _run(cmd):
 
 
 # One object per word class of the enumeration kernels (compiled in parallel).
-PASS_CLASSES = 8
+PASS_CLASSES = 9
 
 
 def _compile(job) -> str:

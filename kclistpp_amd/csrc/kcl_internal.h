@@ -64,9 +64,23 @@ struct DevBuf {
 // Word classes of the enumeration kernels: a root's out-neighbourhood of d
 // nodes is held as d rows of W 64-bit words (d <= 64 W), processed by a
 // workgroup of kClassWaves wavefronts.
-constexpr uint32_t kNumClasses = 8;
-constexpr uint32_t kClassWords[kNumClasses] = {1, 2, 3, 4, 6, 8, 12, 16};
-constexpr uint32_t kClassWaves[kNumClasses] = {1, 2, 4, 4, 4, 4, 4, 4};
+// Waves per workgroup (the default; class_waves below gives the count per
+// class and k): one for the 64-bit class, two for 128 bits, eight wider
+// (the wide classes are LDS-limited to few workgroups per CU, so more lanes
+// per root pay).
+constexpr uint32_t kNumClasses = 9;
+constexpr uint32_t kClassWords[kNumClasses] = {1, 2, 3, 4, 6, 8, 10, 12, 16};
+constexpr uint32_t kClassWaves[kNumClasses] = {1, 2, 8, 8, 8, 8, 8, 8, 8};
+// Waves of the class-c kernel for k (measured per class and k): the
+// 192- to 512-member classes use eight only for k <= 5; the 640-member and
+// wider classes (one or two workgroups per CU by LDS) use eight up to k = 7
+// and four from k = 8, where eight would not leave their deeper loops the
+// registers.
+constexpr uint32_t class_waves(uint32_t c, uint32_t k) {
+  return kClassWords[c] <= 2 ? kClassWaves[c]
+         : kClassWords[c] >= 10 ? (k >= 8 ? 4u : 8u)
+                                : (k <= 5 ? 8u : 4u);
+}
 constexpr uint32_t kMaxWords = 16;
 constexpr uint32_t kMaxRootDegree = 64 * kMaxWords;
 
@@ -159,13 +173,14 @@ kcl_status upload_plan(const PassPlan &plan, DevicePlan *dev);
 // Tasks cheaper than a resident workgroup's share of its class divided by
 // kLightShare are claimed in chunks (light_cost gives that bound).
 constexpr double kLightShare = 64.0;
-double light_cost(double class_total, uint32_t wclass);
+double light_cost(double class_total, uint32_t wclass, uint32_t k);
 // Moves the tasks of classes below min_class into min_class (wider rows:
 // a class-c kernel handles any root with d+ <= 64 kClassWords[c]).
 void promote_plan(PassPlan *plan, uint32_t min_class);
 
-// Concurrent task slots of a word class in one launch (workgroups).
-uint32_t class_slots(uint32_t wclass);
+// Concurrent task slots of a word class's k kernel in one launch
+// (workgroups).
+uint32_t class_slots(uint32_t wclass, uint32_t k);
 
 // Plan for (graph, k): root filter (a root-credit k-pass), costs by the
 // schedule, split tasks.  root_cliques (nullable) receives each root's

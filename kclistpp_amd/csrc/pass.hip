@@ -16,7 +16,8 @@ struct PassTable {
   PassTable() {
     void (*fill[kNumClasses])(PassFn *, PassFn *, size_t *) = {
         fill_pass_class_0, fill_pass_class_1, fill_pass_class_2, fill_pass_class_3,
-        fill_pass_class_4, fill_pass_class_5, fill_pass_class_6, fill_pass_class_7};
+        fill_pass_class_4, fill_pass_class_5, fill_pass_class_6, fill_pass_class_7,
+        fill_pass_class_8};
     for (uint32_t c = 0; c < kNumClasses; ++c) fill[c](fn[c], fn_all[c], &lds[c]);
   }
 };
@@ -39,8 +40,7 @@ uint32_t max_pass_k() { return (uint32_t)kMaxTemplK; }
 
 namespace {
 
-uint32_t blocks_per_cu(uint32_t wclass, const DeviceProps &props) {
-  const uint32_t waves = kClassWaves[wclass];
+uint32_t blocks_per_cu(uint32_t wclass, const DeviceProps &props, uint32_t waves) {
   uint32_t per_cu = (uint32_t)(props.lds_per_cu / (lds_bytes(wclass) + 256));
   if (per_cu < 1) per_cu = 1;
   if (per_cu > 32 / waves) per_cu = 32 / waves;
@@ -60,10 +60,10 @@ __global__ void zero_tickets_kernel(uint32_t *t) {
 
 }  // namespace
 
-uint32_t class_slots(uint32_t wclass) {
+uint32_t class_slots(uint32_t wclass, uint32_t k) {
   DeviceProps props;
   if (wclass >= kNumClasses || device_props(&props) != KCL_OK) return 1;
-  return props.cu_count * blocks_per_cu(wclass, props);
+  return props.cu_count * blocks_per_cu(wclass, props, class_waves(wclass, k));
 }
 
 kcl_status launch_pass(const kcl_graph *g, const uint64_t *d_key, uint32_t k, PassMode mode,
@@ -85,8 +85,8 @@ kcl_status launch_pass(const kcl_graph *g, const uint64_t *d_key, uint32_t k, Pa
       if (!count) continue;
       PassFn fn = kernel_for((uint32_t)c, kk, mode == kCreditAll);
       if (!fn) return fail(KCL_E_UNSUPPORTED, "no enumeration kernel for k = %u", kk);
-      const uint32_t waves = kClassWaves[c];
-      uint32_t per_cu = blocks_per_cu((uint32_t)c, props);
+      const uint32_t waves = class_waves((uint32_t)c, kk);
+      uint32_t per_cu = blocks_per_cu((uint32_t)c, props, waves);
       if (plan.max_per_cu && plan.max_per_cu < per_cu) per_cu = plan.max_per_cu;
       uint64_t blocks = (uint64_t)props.cu_count * per_cu;
       if (blocks > count) blocks = count;
@@ -201,8 +201,8 @@ kcl_status add_edge_tasks(const kcl_graph *g, uint32_t k, const std::vector<uint
   return KCL_OK;
 }
 
-double light_cost(double class_total, uint32_t wclass) {
-  return class_total / ((double)std::max<uint32_t>(1, class_slots(wclass)) * kLightShare);
+double light_cost(double class_total, uint32_t wclass, uint32_t k) {
+  return class_total / ((double)std::max<uint32_t>(1, class_slots(wclass, k)) * kLightShare);
 }
 
 kcl_status build_plan(const kcl_graph *g, uint32_t k, const RootCost *rc, uint32_t split_work,
@@ -239,7 +239,7 @@ kcl_status build_plan(const kcl_graph *g, uint32_t k, const RootCost *rc, uint32
     // workgroup's fair share of its class (and at least split_work), so
     // neither a heavy root nor the rebuilds of many parts set the class's
     // time.  The plain d+^2 schedule keeps the absolute split_work.
-    const double tau = rc ? total[c] / (4.0 * std::max<uint32_t>(1, class_slots(c))) : 0.0;
+    const double tau = rc ? total[c] / (4.0 * std::max<uint32_t>(1, class_slots(c, k))) : 0.0;
     for (Entry &e : cls[c]) {
       const uint32_t d = cd[e.s + 1] - cd[e.s];
       // Each part rebuilds the root's rows, so a part takes at least
@@ -261,7 +261,7 @@ kcl_status build_plan(const kcl_graph *g, uint32_t k, const RootCost *rc, uint32
     std::stable_sort(cls[c].begin(), cls[c].end(),
                      [](const Entry &a, const Entry &b) { return a.cost > b.cost; });
     plan->class_begin[c] = at;
-    const double light = light_cost(total[c], c);
+    const double light = light_cost(total[c], c, k);
     uint32_t cnt = 0;
     plan->light_from[c] = UINT32_MAX;
     for (const Entry &e : cls[c])

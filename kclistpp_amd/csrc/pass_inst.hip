@@ -1,5 +1,5 @@
 // pass_inst.hip - instantiates the enumeration kernels of one word class.
-// Built once per class with -DKCL_PASS_CLASS=<c> (c = 0..7).
+// Built once per class with -DKCL_PASS_CLASS=<c> (c = 0..8).
 #include "pass_kernel.h"
 
 #ifndef KCL_PASS_CLASS
@@ -10,12 +10,14 @@ namespace kcl {
 
 namespace {
 
-template <int W, int NW, int K>
+template <int C, int K>
 struct KTable {
   static void fill(PassFn *t, PassFn *t_all) {
+    constexpr int W = (int)kClassWords[C];
+    constexpr int NW = (int)class_waves(C, K);
     t[K] = &pass_kernel<W, K, NW, false>;
     t_all[K] = &pass_kernel<W, K, NW, true>;
-    if constexpr (K < kMaxTemplK) KTable<W, NW, K + 1>::fill(t, t_all);
+    if constexpr (K < kMaxTemplK) KTable<C, K + 1>::fill(t, t_all);
   }
 };
 
@@ -27,8 +29,7 @@ struct KTable {
 void KCL_CAT(fill_pass_class_, KCL_PASS_CLASS)(PassFn *fn, PassFn *fn_all, size_t *lds) {
   constexpr int C = KCL_PASS_CLASS;
   constexpr int W = (int)kClassWords[C];
-  constexpr int NW = (int)kClassWaves[C];
-  KTable<W, NW, 2>::fill(fn, fn_all);
+  KTable<C, 2>::fill(fn, fn_all);
   *lds = sizeof(Lds<W>);
 }
 

@@ -55,7 +55,7 @@ struct Lds {
   unsigned long long root_sum;  // the root's credits, summed over the waves
   uint32_t task;                // ticket broadcast
   uint32_t next_item;           // counting items handed out so far
-  uint32_t wcount[4];           // edge tasks: members found per wavefront
+  uint32_t wcount[16];          // edge tasks: members found per wavefront
   uint32_t ecount;              // edge tasks: members found so far
   uint32_t chunk;               // 0 while claiming single tasks, else the chunk size
 };
@@ -187,6 +187,7 @@ __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restric
                                                        unsigned long long *__restrict__ cnt,
                                                        unsigned long long *__restrict__ prof) {
   static_assert(K >= 2, "k >= 2");
+  static_assert(NW >= 1 && NW <= 16, "at most 16 wavefronts per workgroup");
   constexpr uint32_t NT = 64 * NW;  // threads of the workgroup sharing one root
   __shared__ Lds<W> sh;
   const uint32_t tid = threadIdx.x;
@@ -626,6 +627,7 @@ void fill_pass_class_4(PassFn *, PassFn *, size_t *);
 void fill_pass_class_5(PassFn *, PassFn *, size_t *);
 void fill_pass_class_6(PassFn *, PassFn *, size_t *);
 void fill_pass_class_7(PassFn *, PassFn *, size_t *);
-static_assert(kNumClasses == 8, "one fill_pass_class_<c> per word class");
+void fill_pass_class_8(PassFn *, PassFn *, size_t *);
+static_assert(kNumClasses == 9, "one fill_pass_class_<c> per word class");
 
 }  // namespace kcl

@@ -112,7 +112,7 @@ kcl_status build_plan_learned(const kcl_graph *g, uint32_t k, const std::vector<
   plan->tasks.clear();
   uint32_t at = 0;
   for (uint32_t c = 0; c < kNumClasses; ++c) {
-    const double slots = (double)std::max<uint32_t>(1, class_slots(c));
+    const double slots = (double)std::max<uint32_t>(1, class_slots(c, k));
     const double tau = total[c] / (slots * kLearnedDepth);
     for (Entry &e : cls[c]) {
       const double target = std::max(tau, (double)kBuildShare * std::max(e.b, 1.0));
@@ -128,7 +128,7 @@ kcl_status build_plan_learned(const kcl_graph *g, uint32_t k, const std::vector<
     std::stable_sort(cls[c].begin(), cls[c].end(),
                      [](const Entry &a, const Entry &b) { return a.cost > b.cost; });
     plan->class_begin[c] = at;
-    const double light = light_cost(total[c], c);
+    const double light = light_cost(total[c], c, k);
     uint32_t cnt = 0;
     plan->light_from[c] = UINT32_MAX;
     for (const Entry &e : cls[c])
