@@ -101,6 +101,13 @@ kcl_status kcl_graph_edges(const kcl_graph *g, uint32_t *st);
  * whose degeneracy exceeds 1024 run (each common out-neighbourhood must stay
  * <= 1024).  Changes timings only; plans made after the call use it. */
 kcl_status kcl_graph_set_edge_split(kcl_graph *g, uint32_t max_vertex_degree);
+/* Counting of roots with dense out-neighbourhoods (k >= 6): a root whose
+ * expected common out-set per edge item, rho^2 d / 3 (d = d+(s), rho the
+ * edge density of G[N+(s)]), is at least min_items is counted a wavefront
+ * per item instead of a lane per item.  Roots of more than 448 members are
+ * always counted that way.  0 = every root; default 64.  Changes timings
+ * only (no reference counterpart); passes launched after the call use it. */
+kcl_status kcl_graph_set_dense_threshold(kcl_graph *g, float min_items);
 /* CountEdges (Graph.h:51, Graph.c:158-166): edges with both ends in nodes[]. */
 kcl_status kcl_count_edges(const kcl_graph *g, uint32_t num_nodes, const uint32_t *nodes,
                            uint32_t *m_out);

@@ -127,6 +127,7 @@ EXPORTS = [
     "kcl_last_error", "kcl_status_name", "kcl_read_edge_list", "kcl_write_edge_list", "kcl_free",
     "kcl_graph_create", "kcl_graph_destroy", "kcl_graph_get_info", "kcl_graph_rank",
     "kcl_graph_set_edge_split",
+    "kcl_graph_set_dense_threshold",
     "kcl_graph_csr", "kcl_graph_edges", "kcl_count_edges", "kcl_clique_enum", "kcl_sim_create",
     "kcl_sim_destroy", "kcl_sim_step", "kcl_sim_counts", "kcl_sim_rho", "kcl_sim_set_rho",
     "kcl_sim_totals", "kcl_sim_last_pass", "kcl_sim_plan_info", "kcl_sim_root_cliques", "kcl_sim_profile_pass", "kcl_sim_find_densest", "kcl_cmpsync_run",
@@ -165,6 +166,7 @@ def lib() -> C.CDLL:
         "kcl_graph_get_info": ([vp, C.POINTER(_GraphInfo)], C.c_int),
         "kcl_graph_rank": ([vp, vp], C.c_int),
         "kcl_graph_set_edge_split": ([vp, u32], C.c_int),
+        "kcl_graph_set_dense_threshold": ([vp, C.c_float], C.c_int),
         "kcl_graph_csr": ([vp, vp, vp], C.c_int),
         "kcl_graph_edges": ([vp, vp], C.c_int),
         "kcl_count_edges": ([vp, u32, vp, C.POINTER(u32)], C.c_int),
@@ -339,6 +341,12 @@ class Graph:
     def set_edge_split(self, max_vertex_degree: int) -> None:
         """Roots with more DAG out-neighbours run as edge tasks (timing only)."""
         _check(lib().kcl_graph_set_edge_split(self.handle, max_vertex_degree))
+
+    def set_dense_threshold(self, min_items: float) -> None:
+        """k >= 6 roots whose expected common out-set per edge item is at
+        least min_items are counted a wavefront per item (0: all; timing
+        only)."""
+        _check(lib().kcl_graph_set_dense_threshold(self.handle, float(min_items)))
 
     def count_edges(self, nodes: np.ndarray) -> int:
         nodes = np.ascontiguousarray(nodes, dtype=np.uint32)

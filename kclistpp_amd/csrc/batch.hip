@@ -320,7 +320,10 @@ kcl_status batch_create(kcl_graph *const *graphs, const uint32_t *ks, uint32_t c
     cd[N] = (uint32_t)M;
   }
   KCL_TRY(graph_from_csr(std::move(cd), std::move(adj), &b->u));
-  for (uint32_t i = 0; i < count; ++i) b->u->edge_above = std::min(b->u->edge_above, graphs[i]->edge_above);
+  for (uint32_t i = 0; i < count; ++i) {
+    b->u->edge_above = std::min(b->u->edge_above, graphs[i]->edge_above);
+    b->u->dense_items = std::min(b->u->dense_items, graphs[i]->dense_items);
+  }
   hipStream_t st = b->u->stream;
   for (int i = 0; i < 2; ++i) KCL_HIP(hipEventCreate(&b->ev[i]));
   KCL_TRY(b->rho.alloc(N));

@@ -320,6 +320,14 @@ extern "C" kcl_status kcl_graph_set_edge_split(kcl_graph *g, uint32_t max_vertex
   return KCL_OK;
 }
 
+extern "C" kcl_status kcl_graph_set_dense_threshold(kcl_graph *g, float min_items) {
+  if (!g) return fail(KCL_E_INVALID_ARG, "NULL argument");
+  if (!(min_items >= 0.0f) || min_items > 1e9f)
+    return fail(KCL_E_INVALID_ARG, "min_items = %g: 0 <= value <= 1e9", (double)min_items);
+  g->dense_items = min_items;
+  return KCL_OK;
+}
+
 extern "C" kcl_status kcl_graph_get_info(const kcl_graph *g, kcl_graph_info *info) {
   if (!g || !info) return fail(KCL_E_INVALID_ARG, "NULL argument");
   info->n_in = g->n_in;

@@ -71,15 +71,16 @@ struct DevBuf {
 constexpr uint32_t kNumClasses = 9;
 constexpr uint32_t kClassWords[kNumClasses] = {1, 2, 3, 4, 6, 8, 10, 12, 16};
 constexpr uint32_t kClassWaves[kNumClasses] = {1, 2, 8, 8, 8, 8, 8, 8, 8};
-// Waves of the class-c kernel for k (measured per class and k): the
-// 192- to 512-member classes use eight only for k <= 5; the 640-member and
-// wider classes (one or two workgroups per CU by LDS) use eight up to k = 7
+// Waves of the class-c kernel for k (measured per class and k; `all`: the
+// k-clique-degree kernels): the narrower classes use eight only for k <= 5;
+// the wide ones (one or two workgroups per CU by LDS) use eight up to k = 7
 // and four from k = 8, where eight would not leave their deeper loops the
-// registers.
-constexpr uint32_t class_waves(uint32_t c, uint32_t k) {
+// registers.  Wide means 512+ members for the counting kernels (dense-only
+// from k = 6, few registers) and 640+ for the k-clique-degree kernels.
+constexpr uint32_t class_waves(uint32_t c, uint32_t k, bool all = false) {
   return kClassWords[c] <= 2 ? kClassWaves[c]
-         : kClassWords[c] >= 10 ? (k >= 8 ? 4u : 8u)
-                                : (k <= 5 ? 8u : 4u);
+         : kClassWords[c] >= (all ? 10u : 8u) ? (k >= 8 ? 4u : 8u)
+                                              : (k <= 5 ? 8u : 4u);
 }
 constexpr uint32_t kMaxWords = 16;
 constexpr uint32_t kMaxRootDegree = 64 * kMaxWords;
@@ -246,5 +247,8 @@ struct kcl_graph {
   uint32_t core_h = 0;
   // Roots with more out-neighbours than this run as edge tasks.
   uint32_t edge_above = kcl::kMaxRootDegree;
+  // k >= 6 roots whose expected common out-set per edge item reaches this
+  // are counted a wavefront per item.
+  float dense_items = 64.0f;
   hipStream_t stream = nullptr;
 };

@@ -85,7 +85,7 @@ kcl_status launch_pass(const kcl_graph *g, const uint64_t *d_key, uint32_t k, Pa
       if (!count) continue;
       PassFn fn = kernel_for((uint32_t)c, kk, mode == kCreditAll);
       if (!fn) return fail(KCL_E_UNSUPPORTED, "no enumeration kernel for k = %u", kk);
-      const uint32_t waves = class_waves((uint32_t)c, kk);
+      const uint32_t waves = class_waves((uint32_t)c, kk, mode == kCreditAll);
       uint32_t per_cu = blocks_per_cu((uint32_t)c, props, waves);
       if (plan.max_per_cu && plan.max_per_cu < per_cu) per_cu = plan.max_per_cu;
       uint64_t blocks = (uint64_t)props.cu_count * per_cu;
@@ -94,7 +94,7 @@ kcl_status launch_pass(const kcl_graph *g, const uint64_t *d_key, uint32_t k, Pa
                          (const uint32_t *)g->d_cd.p, (const uint32_t *)g->d_adj.p, d_key,
                          (const unsigned long long *)(g->core_h ? g->d_core.p : nullptr),
                          g->core_h, plan.tasks.p + begin[c], count,
-                         grp ? count : std::min(plan.light_from[c], count),
+                         grp ? count : std::min(plan.light_from[c], count), g->dense_items,
                          plan.tickets.p + 2 * (grp * kNumClasses + c), (uint32_t)mode, d_cnt,
                          prof ? prof + 2 * (size_t)begin[c] : nullptr);
       KCL_HIP(hipGetLastError());

@@ -14,9 +14,8 @@ template <int C, int K>
 struct KTable {
   static void fill(PassFn *t, PassFn *t_all) {
     constexpr int W = (int)kClassWords[C];
-    constexpr int NW = (int)class_waves(C, K);
-    t[K] = &pass_kernel<W, K, NW, false>;
-    t_all[K] = &pass_kernel<W, K, NW, true>;
+    t[K] = &pass_kernel<W, K, (int)class_waves(C, K), false>;
+    t_all[K] = &pass_kernel<W, K, (int)class_waves(C, K, true), true>;
     if constexpr (K < kMaxTemplK) KTable<C, K + 1>::fill(t, t_all);
   }
 };

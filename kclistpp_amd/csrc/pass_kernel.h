@@ -35,6 +35,15 @@
 // Heavy roots are split into parts: part p of np takes items
 // [p*I/np, (p+1)*I/np) and rebuilds A' (the rebuild is cheap next to the
 // counting it splits).
+//
+// Dense roots (k >= 6).  The k >= 6 kernels number the members by their
+// (key, id) rank before building A', so A'[x] holds only ranks above x
+// (its words below x's own are zero) and the winners are the ranks below
+// nwin.  A root whose expected common out-set per item is large is counted
+// a wavefront per item: the sets of the outer levels are the same in
+// every lane (scalar registers), and in the last two levels lane l takes
+// member 64j + l of each word j and counts its row over words >= j.  Rows
+// of 8 or more words (roots inside a dense core) are always counted so.
 #pragma once
 
 #include "kcl_internal.h"
@@ -58,6 +67,7 @@ struct Lds {
   uint32_t wcount[16];          // edge tasks: members found per wavefront
   uint32_t ecount;              // edge tasks: members found so far
   uint32_t chunk;               // 0 while claiming single tasks, else the chunk size
+  uint32_t nwin;                // ranked kernels: members with key <= the root's key
 };
 
 __device__ __forceinline__ uint32_t select_bit(unsigned long long x, uint32_t r) {
@@ -147,6 +157,72 @@ __device__ __forceinline__ uint64_t cliques_all(const unsigned long long (&P)[W]
   return total;
 }
 
+// Dense counting (ranked kernels).  Members are numbered by (key, id)
+// rank, so the row of member x holds only members ranked after x and, for
+// x in word j, its words below j are zero.  A wavefront works on one set S
+// at a time, the same in every lane (kept in scalar registers), and the
+// lanes share the last level.
+
+template <int W>
+__device__ __forceinline__ void to_sgpr(unsigned long long (&S)[W]) {
+#pragma unroll
+  for (int j = 0; j < W; ++j) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)S[j]);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(S[j] >> 32));
+    S[j] = ((unsigned long long)hi << 32) | lo;
+  }
+}
+
+// This lane's share of the edges inside S: for each word j of S, lane l
+// takes member 64j + l if it is in S and counts its row inside S (words j
+// and above only).
+template <int W>
+__device__ __forceinline__ uint32_t wave_edges(const unsigned long long (&S)[W],
+                                               const unsigned long long *rows, uint32_t lane) {
+  uint32_t c = 0;
+#pragma unroll
+  for (int j = 0; j < W; ++j) {
+    if (S[j] == 0) continue;
+    if ((S[j] >> lane) & 1ull) {
+      const unsigned long long *r = rows + (uint32_t)(64 * j + lane) * W;
+#pragma unroll
+      for (int jj = j; jj < W; ++jj) c += (uint32_t)__popcll(S[jj] & r[jj]);
+    }
+  }
+  return c;
+}
+
+// This lane's share of the J-cliques inside S (J >= 2): the members of S
+// are walked in the same order by every lane, the last two levels go to
+// wave_edges.
+template <int W, int J>
+__device__ __forceinline__ uint64_t wave_cliques(const unsigned long long (&S)[W],
+                                                 const unsigned long long *rows, uint32_t lane) {
+  if constexpr (J == 2) {
+    return wave_edges<W>(S, rows, lane);
+  } else {
+    uint64_t total = 0;
+#pragma unroll 1
+    for (int wi = 0; wi < W; ++wi) {
+      unsigned long long bits = 0;
+#pragma unroll
+      for (int j = 0; j < W; ++j) bits = (j == wi) ? S[j] : bits;
+      while (bits) {
+        const uint32_t w = (uint32_t)wi * 64 + (uint32_t)__builtin_ctzll(bits);
+        bits &= bits - 1;
+        const unsigned long long *r = rows + w * W;
+        unsigned long long Q[W];
+#pragma unroll
+        for (int j = 0; j < W; ++j) Q[j] = (j >= wi) ? (S[j] & r[j]) : 0ull;
+        to_sgpr<W>(Q);
+        if (popc_set<W>(Q) < (uint32_t)(J - 1)) continue;
+        total += wave_cliques<W, J - 1>(Q, rows, lane);
+      }
+    }
+    return total;
+  }
+}
+
 __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
@@ -182,6 +258,7 @@ __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restric
                                                        uint32_t core_h,
                                                        const Task *__restrict__ tasks,
                                                        uint32_t ntasks, uint32_t light_from,
+                                                       float dense_items,
                                                        uint32_t *__restrict__ ticket,
                                                        uint32_t mode,
                                                        unsigned long long *__restrict__ cnt,
@@ -193,6 +270,14 @@ __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restric
   const uint32_t tid = threadIdx.x;
   const uint32_t lane = tid & 63;
   const bool to_root = !ALL && (mode == kCreditRoot);
+  // Ranked kernels (k >= 6) number the members by (key, id) rank before
+  // building the rows; their dense roots are counted a wavefront per item.
+  constexpr bool kRanked = (K >= 6);
+  constexpr bool kDense = kRanked && !ALL;
+  // Rows of 8+ words belong to roots inside a dense core (d+ > 448): those
+  // kernels count every task densely and leave out the per-lane path, whose
+  // registers would halve their occupancy.
+  constexpr bool kDenseOnly = kDense && W >= 8;
 
   // Tickets: tasks [0, light_from) one at a time from ticket[0] (heavy
   // first), then the light tail [light_from, ntasks) in chunks of
@@ -311,6 +396,7 @@ __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restric
     if (tid == 0) {
       sh.pre[0] = 0;
       sh.root_sum = 0;
+      sh.nwin = 0;
     }
     if constexpr (K >= 3) {
       for (uint32_t i = tid; i < d * W; i += NT) sh.rows[i] = 0;
@@ -338,6 +424,23 @@ __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restric
       // adjacency slices.  The other members' slices are scanned: an edge
       // (x, y) with y < x has x outside the core unless both ends are inside.
       if (tid < 64) wave_scan(sh.pre + 1, d, lane);
+      if constexpr (kRanked) {
+        // rank(i) = #{j : (key_j, j) < (key_i, i)} (index order is id order),
+        // kept in cred[i] until the rows are built.  Members with
+        // key <= key_s are the first nwin ranks.
+        uint32_t wins = 0;
+        for (uint32_t i = tid; i < d; i += NT) {
+          const uint64_t ki = sh.key[i];
+          uint32_t r = 0;
+          for (uint32_t j = 0; j < d; ++j) {
+            const uint64_t kj = sh.key[j];
+            r += (kj < ki || (kj == ki && j < i)) ? 1u : 0u;
+          }
+          sh.cred[i] = r;
+          wins += ki <= ks ? 1u : 0u;
+        }
+        if (wins) atomicAdd(&sh.nwin, wins);
+      }
       __syncthreads();
       const uint32_t total = sh.pre[d];
       uint32_t h = 0;
@@ -355,8 +458,16 @@ __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restric
       // latencies overlap instead of adding up.
       constexpr uint32_t kBuildBatch = 8;
       auto set_edge = [&](uint32_t a, uint32_t i) {  // members a < i are adjacent
-        const bool a_first = sh.key[a] <= sh.key[i];  // ties: smaller id (a) first
-        const uint32_t from = a_first ? a : i, to = a_first ? i : a;
+        uint32_t from, to;
+        if constexpr (kRanked) {  // bit positions are ranks
+          const uint32_t ra = (uint32_t)sh.cred[a], ri = (uint32_t)sh.cred[i];
+          from = min(ra, ri);
+          to = max(ra, ri);
+        } else {
+          const bool a_first = sh.key[a] <= sh.key[i];  // ties: smaller id (a) first
+          from = a_first ? a : i;
+          to = a_first ? i : a;
+        }
         atomicOr(&sh.rows[from * W + (to >> 6)], 1ull << (to & 63));
       };
       if (h > 1) {
@@ -421,6 +532,15 @@ __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restric
           if (a < i && sh.node[a] == y) set_edge(a, i);
         }
       }
+      if constexpr (kRanked) {
+        __syncthreads();
+        // From here on members are ranks: beg[r] = the node of rank r.
+        for (uint32_t i = tid; i < d; i += NT) {
+          const uint32_t r = (uint32_t)sh.cred[i];
+          sh.beg[r] = sh.node[i];
+          sh.cred[i] = 0;
+        }
+      }
       __syncthreads();
       if (prof && tid == 0) prof[2 * (size_t)ti] = __builtin_amdgcn_s_memtime() - t_start;
 
@@ -473,39 +593,34 @@ __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restric
         if (tid == 0) sh.next_item = ilo;
         __syncthreads();
 
-        uint32_t u = 0, vsec = 0, wi = 0;
-        unsigned long long P[W];
-        unsigned long long bits = 0;
-        unsigned long long acc = 0;
-        bool live = false;
-        uint32_t ucur = 0;
-
-        // Next item with |P| >= kMinSet, or live = false.  For k <= 5 items
-        // are handed out on demand (one LDS atomic per wavefront and round,
-        // each lane taking the next index), so a lane that drew light items
-        // keeps drawing while another works through a heavy one; for k >= 6,
-        // where an item's own inner loops dominate, lanes take every NT-th
-        // item (measured faster there).  A lane's items are increasing,
-        // which the search for u below relies on.
-        constexpr bool kOnDemand = (K <= 5);
-        uint32_t e = ilo + tid;
-        auto fetch = [&]() {
-          live = false;
-          for (;;) {
-            uint32_t it;
-            if constexpr (kOnDemand) {
-              const unsigned long long act = __ballot(1);
-              const uint32_t leader = (uint32_t)__builtin_ctzll(act);
-              uint32_t base = 0;
-              if (lane == leader) base = atomicAdd(&sh.next_item, (uint32_t)__popcll(act));
-              base = __shfl(base, (int)leader);
-              it = base + (uint32_t)__popcll(act & ((1ull << lane) - 1ull));
-            } else {
-              it = e;
-              e += NT;
-            }
-            if (it >= ihi) return;
-            if constexpr (kEdgeItems) {
+        // Dense roots (ranked kernels, expected |P| of an item ~ rho^2 d / 3
+        // >= dense_items, rho the edge density of the oriented graph on L): a
+        // wavefront per item, the lanes sharing the last level.  Items are
+        // handed out one per wavefront from next_item, so a wavefront's items
+        // are increasing.
+        bool dense = kDenseOnly;
+        if constexpr (kDense && !kDenseOnly) {
+          const float dd = (float)d, ni = (float)nitems;
+          dense = d >= 2 && 4.0f * ni * ni >= 3.0f * dense_items * dd * (dd - 1.0f) * (dd - 1.0f);
+        }
+        if (dense) {
+          if constexpr (kDense) {
+            const uint32_t nwin = sh.nwin;
+            uint32_t ucur = 0;
+            // Per-lane sums of the items of member upend (credited to it when
+            // u changes); items credited to the root go to root_acc at once.
+            uint32_t upend = UINT32_MAX;
+            unsigned long long pend = 0;
+            auto flush = [&]() {
+              const unsigned long long c = wave_sum(pend);
+              if (lane == 0 && c) atomicAdd(&sh.cred[upend], c);
+              pend = 0;
+            };
+            for (;;) {
+              uint32_t it = 0;
+              if (lane == 0) it = atomicAdd(&sh.next_item, 1u);
+              it = (uint32_t)__builtin_amdgcn_readfirstlane((int)it);
+              if (it >= ihi) break;
               uint32_t lo = ucur, hi = d;  // pre[u] <= it < pre[u+1]
               while (hi - lo > 1) {
                 const uint32_t mid = (lo + hi) >> 1;
@@ -513,7 +628,7 @@ __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restric
                 else hi = mid;
               }
               ucur = lo;
-              u = lo;
+              const uint32_t u = lo;
               uint32_t r = it - sh.pre[u];
               uint32_t v = 0;
               bool found = false;
@@ -530,58 +645,136 @@ __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restric
                   }
                 }
               }
+              unsigned long long P[W];
 #pragma unroll
               for (int j = 0; j < W; ++j) P[j] = sh.rows[u * W + j] & sh.rows[v * W + j];
-              vsec = v;
-            } else {
-              u = it;
-#pragma unroll
-              for (int j = 0; j < W; ++j) P[j] = sh.rows[u * W + j];
-            }
-            if (popc_set<W>(P) >= kMinSet) {
-              wi = 0;
-              bits = P[0];
-              live = true;
-              return;
-            }
-          }
-        };
-
-        fetch();
-        while (live) {
-          if (bits) {
-            const uint32_t w = wi * 64 + (uint32_t)__builtin_ctzll(bits);
-            bits &= bits - 1;
-            const unsigned long long *r = sh.rows + w * W;
-            unsigned long long Q[W];
-#pragma unroll
-            for (int j = 0; j < W; ++j) Q[j] = P[j] & r[j];
-            if constexpr (ALL) {
-              const uint64_t c = cliques_all<W, kStepDepth>(Q, sh.rows, sh.cred);
-              if (c) atomicAdd(&sh.cred[w], (unsigned long long)c);
-              acc += c;
-            } else {
-              acc += cliques_in<W, kStepDepth>(Q, sh.rows);
-            }
-          } else if (wi + 1 < (uint32_t)W) {
-            ++wi;
-            unsigned long long nb = 0;
-#pragma unroll
-            for (int j = 0; j < W; ++j) nb = (j == (int)wi) ? P[j] : nb;
-            bits = nb;
-          } else {
-            if (acc) {
-              if constexpr (ALL) {
-                atomicAdd(&sh.cred[u], acc);
-                if constexpr (kEdgeItems) atomicAdd(&sh.cred[vsec], acc);
-                root_acc += acc;
+              to_sgpr<W>(P);
+              if (popc_set<W>(P) < (uint32_t)(K - 3)) continue;
+              const uint64_t c = wave_cliques<W, K - 3>(P, sh.rows, lane);
+              if (to_root || u >= nwin) {
+                root_acc += c;
               } else {
-                if (!to_root && sh.key[u] <= ks) atomicAdd(&sh.cred[u], acc);
-                else root_acc += acc;
+                if (u != upend) {
+                  if (upend != UINT32_MAX) flush();
+                  upend = u;
+                }
+                pend += c;
               }
-              acc = 0;
             }
-            fetch();
+            if (upend != UINT32_MAX) flush();
+          }
+        } else if constexpr (!kDenseOnly) {
+          uint32_t u = 0, vsec = 0, wi = 0;
+          unsigned long long P[W];
+          unsigned long long bits = 0;
+          unsigned long long acc = 0;
+          bool live = false;
+          uint32_t ucur = 0;
+
+          // Next item with |P| >= kMinSet, or live = false.  For k <= 5 items
+          // are handed out on demand (one LDS atomic per wavefront and round,
+          // each lane taking the next index), so a lane that drew light items
+          // keeps drawing while another works through a heavy one; for k >= 6,
+          // where an item's own inner loops dominate, lanes take every NT-th
+          // item (measured faster there).  A lane's items are increasing,
+          // which the search for u below relies on.
+          constexpr bool kOnDemand = (K <= 5);
+          uint32_t e = ilo + tid;
+          auto fetch = [&]() {
+            live = false;
+            for (;;) {
+              uint32_t it;
+              if constexpr (kOnDemand) {
+                const unsigned long long act = __ballot(1);
+                const uint32_t leader = (uint32_t)__builtin_ctzll(act);
+                uint32_t base = 0;
+                if (lane == leader) base = atomicAdd(&sh.next_item, (uint32_t)__popcll(act));
+                base = __shfl(base, (int)leader);
+                it = base + (uint32_t)__popcll(act & ((1ull << lane) - 1ull));
+              } else {
+                it = e;
+                e += NT;
+              }
+              if (it >= ihi) return;
+              if constexpr (kEdgeItems) {
+                uint32_t lo = ucur, hi = d;  // pre[u] <= it < pre[u+1]
+                while (hi - lo > 1) {
+                  const uint32_t mid = (lo + hi) >> 1;
+                  if (sh.pre[mid] <= it) lo = mid;
+                  else hi = mid;
+                }
+                ucur = lo;
+                u = lo;
+                uint32_t r = it - sh.pre[u];
+                uint32_t v = 0;
+                bool found = false;
+#pragma unroll
+                for (int j = 0; j < W; ++j) {
+                  const unsigned long long word = sh.rows[u * W + j];
+                  const uint32_t c = __popcll(word);
+                  if (!found) {
+                    if (r < c) {
+                      v = (uint32_t)j * 64 + select_bit(word, r);
+                      found = true;
+                    } else {
+                      r -= c;
+                    }
+                  }
+                }
+#pragma unroll
+                for (int j = 0; j < W; ++j) P[j] = sh.rows[u * W + j] & sh.rows[v * W + j];
+                vsec = v;
+              } else {
+                u = it;
+#pragma unroll
+                for (int j = 0; j < W; ++j) P[j] = sh.rows[u * W + j];
+              }
+              if (popc_set<W>(P) >= kMinSet) {
+                wi = 0;
+                bits = P[0];
+                live = true;
+                return;
+              }
+            }
+          };
+
+          fetch();
+          while (live) {
+            if (bits) {
+              const uint32_t w = wi * 64 + (uint32_t)__builtin_ctzll(bits);
+              bits &= bits - 1;
+              const unsigned long long *r = sh.rows + w * W;
+              unsigned long long Q[W];
+#pragma unroll
+              for (int j = 0; j < W; ++j) Q[j] = P[j] & r[j];
+              if constexpr (ALL) {
+                const uint64_t c = cliques_all<W, kStepDepth>(Q, sh.rows, sh.cred);
+                if (c) atomicAdd(&sh.cred[w], (unsigned long long)c);
+                acc += c;
+              } else {
+                acc += cliques_in<W, kStepDepth>(Q, sh.rows);
+              }
+            } else if (wi + 1 < (uint32_t)W) {
+              ++wi;
+              unsigned long long nb = 0;
+#pragma unroll
+              for (int j = 0; j < W; ++j) nb = (j == (int)wi) ? P[j] : nb;
+              bits = nb;
+            } else {
+              if (acc) {
+                if constexpr (ALL) {
+                  atomicAdd(&sh.cred[u], acc);
+                  if constexpr (kEdgeItems) atomicAdd(&sh.cred[vsec], acc);
+                  root_acc += acc;
+                } else {
+                  const bool wins = kRanked ? (u < sh.nwin) : (sh.key[u] <= ks);
+                  if (!to_root && wins) atomicAdd(&sh.cred[u], acc);
+                  else root_acc += acc;
+                }
+                acc = 0;
+              }
+              fetch();
+            }
           }
         }
       }
@@ -603,7 +796,7 @@ __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restric
     if (prof && tid == 0) prof[2 * (size_t)ti + 1] = __builtin_amdgcn_s_memtime() - t_start;
     for (uint32_t i = tid; i < d; i += NT) {
       const unsigned long long c = sh.cred[i];
-      if (c) atomicAdd(&cnt[sh.node[i]], c);
+      if (c) atomicAdd(&cnt[kRanked ? sh.beg[i] : sh.node[i]], c);
     }
     __syncthreads();
   }
@@ -613,7 +806,7 @@ __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restric
 
 using PassFn = void (*)(const uint32_t *, const uint32_t *, const uint64_t *,
                         const unsigned long long *, uint32_t, const Task *, uint32_t, uint32_t,
-                        uint32_t *, uint32_t, unsigned long long *, unsigned long long *);
+                        float, uint32_t *, uint32_t, unsigned long long *, unsigned long long *);
 
 constexpr int kMaxTemplK = 10;  // k handled by compiled kernels: 2..kMaxTemplK
 

@@ -195,6 +195,67 @@ def test_wide_neighbourhoods_vs_oracle(kcl, oracle, part, k):
         assert visits == wv and np.array_equal(acc, want.astype(np.float64))
 
 
+@pytest.mark.parametrize("size,p,k,above", [
+    (40, 0.8, 6, 1024), (40, 0.8, 8, 1024), (40, 0.8, 10, 1024),  # 64-member rows
+    (200, 0.6, 6, 1024), (200, 0.6, 7, 1024),                      # 128 (core ~104)
+    (500, 0.3, 8, 1024),                                           # 192 (core ~130)
+    (800, 0.33, 6, 1024), (800, 0.33, 9, 1024),                    # 256 (core ~236)
+    (200, 0.6, 7, 40),                                             # edge tasks
+])
+def test_dense_counting_vs_oracle(kcl, oracle, size, p, k, above):
+    """Every root counted a wavefront per item (threshold 0): members ranked
+    by (key, id), the winner as a rank prefix, the lanes sharing the last
+    level, rows of one to four words; with many key ties."""
+    pairs = _community_graph(size + k, size=size, p=p)
+    og = oracle.dag(pairs)
+    rng = np.random.default_rng(k)
+    rho = rng.integers(0, 4, og.n).astype(np.float64) * 0.25
+    level = rng.permutation(og.n).astype(np.uint32)
+    _say("dense size", size, "k", k, "core", og.core, "edge split", above)
+    with kcl.Graph(pairs) as g:
+        g.set_dense_threshold(0)
+        g.set_edge_split(above)
+        acc, visits = g.clique_enum(k, kcl.FRANK_WOLFE, rho=rho)
+        want, wv = oracle.fw_counts(og, k, rho)
+        assert visits == wv and np.array_equal(acc, want.astype(np.float64))
+        acc, _ = g.clique_enum(k, kcl.PAVA_PREPROCESS, level=level)
+        assert np.array_equal(acc, oracle.pava_counts(og, k, level).astype(np.float64))
+
+
+@pytest.mark.parametrize("k", [6, 7])
+def test_wide_dense_rows_vs_oracle(kcl, oracle, k):
+    """Rows of 10 words at k >= 6 are always counted densely, here on
+    sparse out-neighbourhoods (K_{560,560} plus sparse edges per side)."""
+    pairs = _bipartite_plus(560, 0.02, 600 + k)
+    og = oracle.dag(pairs)
+    assert og.core > 512
+    rng = np.random.default_rng(11)
+    rho = rng.integers(0, 3, og.n).astype(np.float64)
+    _say("wide dense", k, "core", og.core)
+    with kcl.Graph(pairs) as g:
+        acc, visits = g.clique_enum(k, kcl.FRANK_WOLFE, rho=rho)
+        want, wv = oracle.fw_counts(og, k, rho)
+        assert visits == wv and np.array_equal(acc, want.astype(np.float64))
+
+
+def test_counts_do_not_depend_on_dense_threshold(kcl):
+    pairs = _community_graph(11, size=200, p=0.6)
+    _say("dense threshold in sim")
+    with kcl.Graph(pairs) as g, kcl.Graph(pairs) as h:
+        h.set_dense_threshold(0)
+        with pytest.raises(kcl.KclError):
+            h.set_dense_threshold(-1)
+        a, b = kcl.Sim(g, 6), kcl.Sim(h, 6, split_work=7)
+        for t in (1, 2, 3):
+            a.step(t)
+            b.step(t)
+            assert np.array_equal(a.counts(), b.counts())
+            assert np.array_equal(a.rho().view(np.uint64), b.rho().view(np.uint64))
+        assert a.find_densest() == b.find_densest()
+        a.close()
+        b.close()
+
+
 @pytest.mark.parametrize("split", [1, 7, 100])
 def test_counts_do_not_depend_on_splitting(kcl, split):
     pairs = _community_graph(5, size=60, p=0.9)
