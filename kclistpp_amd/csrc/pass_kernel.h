@@ -52,10 +52,19 @@ namespace kcl {
 
 namespace {
 
+// Words between consecutive rows in LDS: odd for the even classes below 16
+// words, so that 64 lanes reading one word of consecutive rows hit distinct
+// banks (an even stride of 2^a * odd words repeats banks every 16 / 2^a
+// lanes).  The 16-word class stays unpadded (it would not fit).
+template <int W>
+constexpr int row_stride() {
+  return (W % 2 == 0 && W < 16) ? W + 1 : W;
+}
+
 template <int W>
 struct Lds {
   static constexpr int D = 64 * W;
-  unsigned long long rows[D * W];  // oriented rows A'
+  unsigned long long rows[D * row_stride<W>()];  // oriented rows A'
   uint64_t key[D];                 // member keys
   unsigned long long cred[D];   // credits gathered per member
   uint32_t node[D];             // member ids, ascending
@@ -111,7 +120,7 @@ __device__ __forceinline__ uint64_t cliques_in(const unsigned long long (&P)[W],
       while (bits) {
         const uint32_t b = (uint32_t)__builtin_ctzll(bits);
         bits &= bits - 1;
-        const unsigned long long *r = rows + (uint32_t)(wi * 64 + b) * W;
+        const unsigned long long *r = rows + (uint32_t)(wi * 64 + b) * row_stride<W>();
         unsigned long long Q[W];
 #pragma unroll
         for (int j = 0; j < W; ++j) Q[j] = P[j] & r[j];
@@ -143,7 +152,7 @@ __device__ __forceinline__ uint64_t cliques_all(const unsigned long long (&P)[W]
         atomicAdd(&cred[x], 1ull);
         ++total;
       } else {
-        const unsigned long long *r = rows + x * W;
+        const unsigned long long *r = rows + x * row_stride<W>();
         unsigned long long Q[W];
 #pragma unroll
         for (int j = 0; j < W; ++j) Q[j] = P[j] & r[j];
@@ -184,7 +193,7 @@ __device__ __forceinline__ uint32_t wave_edges(const unsigned long long (&S)[W],
   for (int j = 0; j < W; ++j) {
     if (S[j] == 0) continue;
     if ((S[j] >> lane) & 1ull) {
-      const unsigned long long *r = rows + (uint32_t)(64 * j + lane) * W;
+      const unsigned long long *r = rows + (uint32_t)(64 * j + lane) * row_stride<W>();
 #pragma unroll
       for (int jj = j; jj < W; ++jj) c += (uint32_t)__popcll(S[jj] & r[jj]);
     }
@@ -210,7 +219,7 @@ __device__ __forceinline__ uint64_t wave_cliques(const unsigned long long (&S)[W
       while (bits) {
         const uint32_t w = (uint32_t)wi * 64 + (uint32_t)__builtin_ctzll(bits);
         bits &= bits - 1;
-        const unsigned long long *r = rows + w * W;
+        const unsigned long long *r = rows + w * row_stride<W>();
         unsigned long long Q[W];
 #pragma unroll
         for (int j = 0; j < W; ++j) Q[j] = (j >= wi) ? (S[j] & r[j]) : 0ull;
@@ -266,6 +275,7 @@ __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restric
   static_assert(K >= 2, "k >= 2");
   static_assert(NW >= 1 && NW <= 16, "at most 16 wavefronts per workgroup");
   constexpr uint32_t NT = 64 * NW;  // threads of the workgroup sharing one root
+  constexpr uint32_t RS = (uint32_t)row_stride<W>();
   __shared__ Lds<W> sh;
   const uint32_t tid = threadIdx.x;
   const uint32_t lane = tid & 63;
@@ -399,7 +409,7 @@ __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restric
       sh.nwin = 0;
     }
     if constexpr (K >= 3) {
-      for (uint32_t i = tid; i < d * W; i += NT) sh.rows[i] = 0;
+      for (uint32_t i = tid; i < d * RS; i += NT) sh.rows[i] = 0;
     }
     __syncthreads();
 
@@ -468,7 +478,7 @@ __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restric
           from = a_first ? a : i;
           to = a_first ? i : a;
         }
-        atomicOr(&sh.rows[from * W + (to >> 6)], 1ull << (to & 63));
+        atomicOr(&sh.rows[from * RS + (to >> 6)], 1ull << (to & 63));
       };
       if (h > 1) {
         const uint32_t pairs = h * (h - 1) / 2;
@@ -550,14 +560,14 @@ __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restric
         for (uint32_t u = lo + tid; u < hi; u += NT) {
           uint32_t c = 0;
 #pragma unroll
-          for (int j = 0; j < W; ++j) c += __popcll(sh.rows[u * W + j]);
+          for (int j = 0; j < W; ++j) c += __popcll(sh.rows[u * RS + j]);
           if constexpr (ALL) {
             if (c) {
               atomicAdd(&sh.cred[u], (unsigned long long)c);
               root_acc += c;
 #pragma unroll
               for (int j = 0; j < W; ++j) {
-                unsigned long long bits = sh.rows[u * W + j];
+                unsigned long long bits = sh.rows[u * RS + j];
                 while (bits) {
                   const uint32_t x = (uint32_t)j * 64 + (uint32_t)__builtin_ctzll(bits);
                   bits &= bits - 1;
@@ -580,7 +590,7 @@ __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restric
           for (uint32_t u = tid; u < d; u += NT) {
             uint32_t c = 0;
 #pragma unroll
-            for (int j = 0; j < W; ++j) c += __popcll(sh.rows[u * W + j]);
+            for (int j = 0; j < W; ++j) c += __popcll(sh.rows[u * RS + j]);
             sh.pre[u + 1] = c;
           }
           __syncthreads();
@@ -634,7 +644,7 @@ __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restric
               bool found = false;
 #pragma unroll
               for (int j = 0; j < W; ++j) {
-                const unsigned long long word = sh.rows[u * W + j];
+                const unsigned long long word = sh.rows[u * RS + j];
                 const uint32_t c = __popcll(word);
                 if (!found) {
                   if (r < c) {
@@ -647,7 +657,7 @@ __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restric
               }
               unsigned long long P[W];
 #pragma unroll
-              for (int j = 0; j < W; ++j) P[j] = sh.rows[u * W + j] & sh.rows[v * W + j];
+              for (int j = 0; j < W; ++j) P[j] = sh.rows[u * RS + j] & sh.rows[v * RS + j];
               to_sgpr<W>(P);
               if (popc_set<W>(P) < (uint32_t)(K - 3)) continue;
               const uint64_t c = wave_cliques<W, K - 3>(P, sh.rows, lane);
@@ -710,7 +720,7 @@ __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restric
                 bool found = false;
 #pragma unroll
                 for (int j = 0; j < W; ++j) {
-                  const unsigned long long word = sh.rows[u * W + j];
+                  const unsigned long long word = sh.rows[u * RS + j];
                   const uint32_t c = __popcll(word);
                   if (!found) {
                     if (r < c) {
@@ -722,12 +732,12 @@ __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restric
                   }
                 }
 #pragma unroll
-                for (int j = 0; j < W; ++j) P[j] = sh.rows[u * W + j] & sh.rows[v * W + j];
+                for (int j = 0; j < W; ++j) P[j] = sh.rows[u * RS + j] & sh.rows[v * RS + j];
                 vsec = v;
               } else {
                 u = it;
 #pragma unroll
-                for (int j = 0; j < W; ++j) P[j] = sh.rows[u * W + j];
+                for (int j = 0; j < W; ++j) P[j] = sh.rows[u * RS + j];
               }
               if (popc_set<W>(P) >= kMinSet) {
                 wi = 0;
@@ -743,7 +753,7 @@ __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restric
             if (bits) {
               const uint32_t w = wi * 64 + (uint32_t)__builtin_ctzll(bits);
               bits &= bits - 1;
-              const unsigned long long *r = sh.rows + w * W;
+              const unsigned long long *r = sh.rows + w * RS;
               unsigned long long Q[W];
 #pragma unroll
               for (int j = 0; j < W; ++j) Q[j] = P[j] & r[j];
