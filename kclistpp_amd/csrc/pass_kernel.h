@@ -182,9 +182,11 @@ __device__ __forceinline__ void to_sgpr(unsigned long long (&S)[W]) {
   }
 }
 
-// This lane's share of the edges inside S: for each word j of S, lane l
-// takes member 64j + l if it is in S and counts its row inside S (words j
-// and above only).
+// This lane's share of the edges inside S: for each non-zero word j of S,
+// lane l counts the row of member 64j + l inside S (words j and above
+// only) and keeps the count if that member is in S.  Every lane loads (rows
+// past the root's members hold stale bits and are masked out the same
+// way), so the loads of successive words need no branch between them.
 template <int W>
 __device__ __forceinline__ uint32_t wave_edges(const unsigned long long (&S)[W],
                                                const unsigned long long *rows, uint32_t lane) {
@@ -192,11 +194,11 @@ __device__ __forceinline__ uint32_t wave_edges(const unsigned long long (&S)[W],
 #pragma unroll
   for (int j = 0; j < W; ++j) {
     if (S[j] == 0) continue;
-    if ((S[j] >> lane) & 1ull) {
-      const unsigned long long *r = rows + (uint32_t)(64 * j + lane) * row_stride<W>();
+    const unsigned long long *r = rows + (uint32_t)(64 * j + lane) * row_stride<W>();
+    uint32_t cj = 0;
 #pragma unroll
-      for (int jj = j; jj < W; ++jj) c += (uint32_t)__popcll(S[jj] & r[jj]);
-    }
+    for (int jj = j; jj < W; ++jj) cj += (uint32_t)__popcll(S[jj] & r[jj]);
+    c += ((S[j] >> lane) & 1ull) ? cj : 0u;
   }
   return c;
 }
