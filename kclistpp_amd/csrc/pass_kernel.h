@@ -36,8 +36,8 @@
 // [p*I/np, (p+1)*I/np) and rebuilds A' (the rebuild is cheap next to the
 // counting it splits).
 //
-// Dense roots (k >= 6).  The k >= 6 kernels number the members by their
-// (key, id) rank before building A', so A'[x] holds only ranks above x
+// Dense roots (k >= 6).  The k >= 6 kernels of rows of 4+ words number
+// the members by their (key, id) rank before building A', so A'[x] holds only ranks above x
 // (its words below x's own are zero) and the winners are the ranks below
 // nwin.  A root whose expected common out-set per item is large is counted
 // a wavefront per item: the sets of the outer levels are the same in
@@ -282,9 +282,11 @@ __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restric
   const uint32_t tid = threadIdx.x;
   const uint32_t lane = tid & 63;
   const bool to_root = !ALL && (mode == kCreditRoot);
-  // Ranked kernels (k >= 6) number the members by (key, id) rank before
-  // building the rows; their dense roots are counted a wavefront per item.
-  constexpr bool kRanked = (K >= 6);
+  // Ranked kernels (k >= 6, rows of 4+ words) number the members by
+  // (key, id) rank before building the rows; their dense roots are counted
+  // a wavefront per item.  Narrower rows (d <= 192) hardly ever reach the
+  // dense threshold, so they skip the ranking.
+  constexpr bool kRanked = (K >= 6 && W >= 4);
   constexpr bool kDense = kRanked && !ALL;
   // Rows of 8+ words belong to roots inside a dense core (d+ > 448): those
   // kernels count every task densely and leave out the per-lane path, whose
@@ -294,7 +296,7 @@ __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restric
   // Tickets: tasks [0, light_from) one at a time from ticket[0] (heavy
   // first), then the light tail [light_from, ntasks) in chunks of
   // kLightChunk from ticket[1] (one same-address atomic per chunk).
-  constexpr uint32_t kLightChunk = 4;
+  constexpr uint32_t kLightChunk = 8;
   uint32_t chunk_base = 0, chunk_left = 0, chunk_size = 1;
   if (tid == 0) sh.chunk = 0;
   __syncthreads();

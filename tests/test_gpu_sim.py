@@ -200,12 +200,12 @@ def test_wide_neighbourhoods_vs_oracle(kcl, oracle, part, k):
     (200, 0.6, 6, 1024), (200, 0.6, 7, 1024),                      # 128 (core ~104)
     (500, 0.3, 8, 1024),                                           # 192 (core ~130)
     (800, 0.33, 6, 1024), (800, 0.33, 9, 1024),                    # 256 (core ~236)
-    (200, 0.6, 7, 40),                                             # edge tasks
 ])
 def test_dense_counting_vs_oracle(kcl, oracle, size, p, k, above):
-    """Every root counted a wavefront per item (threshold 0): members ranked
-    by (key, id), the winner as a rank prefix, the lanes sharing the last
-    level, rows of one to four words; with many key ties."""
+    """Threshold 0: every root of rows of 4+ words counted a wavefront per
+    item (members ranked by (key, id), the winner as a rank prefix, the
+    lanes sharing the last level); narrower rows keep the per-lane path.
+    Many key ties."""
     pairs = _community_graph(size + k, size=size, p=p)
     og = oracle.dag(pairs)
     rng = np.random.default_rng(k)
@@ -220,6 +220,21 @@ def test_dense_counting_vs_oracle(kcl, oracle, size, p, k, above):
         assert visits == wv and np.array_equal(acc, want.astype(np.float64))
         acc, _ = g.clique_enum(k, kcl.PAVA_PREPROCESS, level=level)
         assert np.array_equal(acc, oracle.pava_counts(og, k, level).astype(np.float64))
+
+
+def test_dense_edge_tasks_match_vertex_tasks(kcl):
+    """A 350-member community with p = 0.9 at k = 7 (1.5e13 cliques, past
+    the oracle's reach): edge tasks of ~250 common out-neighbours, counted
+    densely by the ranked k = 6 kernel, against the vertex tasks."""
+    pairs = _community_graph(21, size=350, p=0.9)
+    rng = np.random.default_rng(5)
+    _say("dense edge tasks k 7")
+    with kcl.Graph(pairs) as g, kcl.Graph(pairs) as ge:
+        rho = rng.integers(0, 4, g.n).astype(np.float64) * 0.25
+        ge.set_edge_split(200)
+        a, va = g.clique_enum(7, kcl.FRANK_WOLFE, rho=rho)
+        b, vb = ge.clique_enum(7, kcl.FRANK_WOLFE, rho=rho)
+        assert va == vb and va > 10 ** 13 and np.array_equal(a, b)
 
 
 @pytest.mark.parametrize("k", [6, 7])
