@@ -202,6 +202,22 @@ static kcl_status build_device_csr(kcl_graph *g, const uint32_t *pairs) {
 
 using kcl::fail;
 
+namespace {
+
+// The graph's stream, its side streams and the pass fork/join events (all
+// default priority, non-blocking).
+hipError_t create_streams(kcl_graph *g) {
+  hipError_t e = hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking);
+  for (int i = 0; e == hipSuccess && i < kcl_graph::kSideStreams; ++i) {
+    e = hipStreamCreateWithFlags(&g->side[i], hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&g->join[i], hipEventDisableTiming);
+  }
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&g->fork, hipEventDisableTiming);
+  return e;
+}
+
+}  // namespace
+
 extern "C" kcl_status kcl_graph_create(const uint32_t *pairs, uint64_t m, uint32_t n_in,
                                        kcl_graph **out) {
   return kcl::graph_create(pairs, m, n_in, nullptr, out);
@@ -239,9 +255,9 @@ kcl_status kcl::graph_create(const uint32_t *pairs, uint64_t m, uint32_t n_in,
     memcpy(g->rank.data(), given_rank, (size_t)n_in * sizeof(uint32_t));
   else
     kcl::degeneracy_rank(n_in, m, pairs, g->rank.data());
-  hipError_t e = hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking);
+  hipError_t e = create_streams(g);
   if (e != hipSuccess) {
-    delete g;
+    kcl_graph_destroy(g);
     return fail(KCL_E_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
   }
   if (m == 0) {
@@ -285,7 +301,7 @@ kcl_status kcl::graph_from_csr(std::vector<uint32_t> &&cd, std::vector<uint32_t>
   g->cd = std::move(cd);
   g->adj = std::move(adj);
   kcl_status st = KCL_OK;
-  hipError_t e = hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking);
+  hipError_t e = create_streams(g);
   if (e != hipSuccess) st = fail(KCL_E_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
   if (st == KCL_OK) st = g->d_cd.alloc(g->n + 1);
   if (st == KCL_OK) st = g->d_adj.alloc(g->m);
@@ -308,6 +324,11 @@ extern "C" void kcl_graph_destroy(kcl_graph *g) {
   g->d_cd.release();
   g->d_adj.release();
   if (g->stream) (void)hipStreamDestroy(g->stream);
+  for (int i = 0; i < kcl_graph::kSideStreams; ++i) {
+    if (g->side[i]) (void)hipStreamDestroy(g->side[i]);
+    if (g->join[i]) (void)hipEventDestroy(g->join[i]);
+  }
+  if (g->fork) (void)hipEventDestroy(g->fork);
   delete g;
 }
 

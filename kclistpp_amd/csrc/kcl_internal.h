@@ -251,4 +251,11 @@ struct kcl_graph {
   // are counted a wavefront per item.
   float dense_items = 64.0f;
   hipStream_t stream = nullptr;
+  // Side streams of a pass's class launches: the pass forks from its
+  // stream with `fork` and joins back with `join` (one host thread uses a
+  // graph at a time, so one set serves every pass on it).
+  static constexpr int kSideStreams = 2;
+  hipStream_t side[kSideStreams] = {};
+  hipEvent_t fork = nullptr;
+  hipEvent_t join[kSideStreams] = {};
 };

@@ -75,6 +75,14 @@ kcl_status launch_pass(const kcl_graph *g, const uint64_t *d_key, uint32_t k, Pa
   static_assert(kTickets <= 64, "tickets of one pass fit one wavefront");
   zero_tickets_kernel<<<1, 64, 0, stream>>>(plan.tickets.p);
   KCL_HIP(hipGetLastError());
+  // The class launches go round-robin to the pass's stream and the graph's
+  // side streams, so a class's tail overlaps the next class (credits are
+  // atomic sums, tickets are per class); the pass's stream joins them all.
+  constexpr int kQueues = 1 + kcl_graph::kSideStreams;
+  KCL_HIP(hipEventRecord(g->fork, stream));
+  for (int i = 0; i < kcl_graph::kSideStreams; ++i)
+    KCL_HIP(hipStreamWaitEvent(g->side[i], g->fork, 0));
+  uint32_t launches = 0;
   // Vertex tasks with the k kernels, then edge tasks with the (k-1) ones.
   for (uint32_t grp = 0; grp < 2; ++grp) {
     const uint32_t *begin = grp ? plan.eclass_begin : plan.class_begin;
@@ -90,7 +98,9 @@ kcl_status launch_pass(const kcl_graph *g, const uint64_t *d_key, uint32_t k, Pa
       if (plan.max_per_cu && plan.max_per_cu < per_cu) per_cu = plan.max_per_cu;
       uint64_t blocks = (uint64_t)props.cu_count * per_cu;
       if (blocks > count) blocks = count;
-      hipLaunchKernelGGL(fn, dim3((uint32_t)blocks), dim3(64 * waves), 0, stream,
+      const int q = (int)(launches++ % kQueues);
+      hipLaunchKernelGGL(fn, dim3((uint32_t)blocks), dim3(64 * waves), 0,
+                         q ? g->side[q - 1] : stream,
                          (const uint32_t *)g->d_cd.p, (const uint32_t *)g->d_adj.p, d_key,
                          (const unsigned long long *)(g->core_h ? g->d_core.p : nullptr),
                          g->core_h, plan.tasks.p + begin[c], count,
@@ -99,6 +109,10 @@ kcl_status launch_pass(const kcl_graph *g, const uint64_t *d_key, uint32_t k, Pa
                          prof ? prof + 2 * (size_t)begin[c] : nullptr);
       KCL_HIP(hipGetLastError());
     }
+  }
+  for (int i = 0; i < kcl_graph::kSideStreams; ++i) {
+    KCL_HIP(hipEventRecord(g->join[i], g->side[i]));
+    KCL_HIP(hipStreamWaitEvent(stream, g->join[i], 0));
   }
   return KCL_OK;
 }
