@@ -92,19 +92,10 @@ def build_clis() -> list[str]:
     return out
 
 
-def build_probe() -> str:
-    """Measurement helper (copy kernel for counter scaling, LDS bandwidth)."""
-    os.makedirs(BIN, exist_ok=True)
-    exe = os.path.join(BIN, "probe")
-    _run([HIPCC, *HIP_FLAGS, "-x", "hip", os.path.join(CSRC, "diag", "probe.hip"), "-o", exe])
-    return exe
-
-
 def build(lib_only: bool = False) -> None:
     build_lib()
     if not lib_only:
         build_clis()
-        build_probe()
 
 
 if __name__ == "__main__":

@@ -198,6 +198,21 @@ struct kcl_batch {
 namespace kcl {
 namespace {
 
+// Enqueues copies of every plan's pass status word into out (read after
+// the stream's next synchronisation, then checked with check_status).
+kcl_status enqueue_status(kcl_batch *b, std::vector<uint32_t> &out, hipStream_t st) {
+  out.assign(b->plans.size(), 0);
+  for (size_t i = 0; i < b->plans.size(); ++i)
+    KCL_HIP(hipMemcpyAsync(&out[i], pass_status_word(b->plans[i]->dev), 4, hipMemcpyDeviceToHost,
+                           st));
+  return KCL_OK;
+}
+
+kcl_status check_status(const std::vector<uint32_t> &w) {
+  for (uint32_t x : w) KCL_TRY(pass_status_error(x));
+  return KCL_OK;
+}
+
 kcl_status launch_all(kcl_batch *b, const uint64_t *key) {
   for (auto &kp : b->plans)
     KCL_TRY(launch_pass(b->u, key, kp->k, kCreditWinner, kp->dev, b->cnt.p, b->u->stream));
@@ -224,7 +239,10 @@ kcl_status time_pass(kcl_batch *b, float *ms) {
     KCL_TRY(launch_all(b, (const uint64_t *)b->rho.p));
     KCL_HIP(hipEventRecord(b->ev[1], st));
     KCL_TRY(zero_u64(b->cnt.p, b->nodes(), st));
+    std::vector<uint32_t> status;
+    KCL_TRY(enqueue_status(b, status, st));
     KCL_HIP(hipStreamSynchronize(st));
+    KCL_TRY(check_status(status));
     float t = 0;
     KCL_HIP(hipEventElapsedTime(&t, b->ev[0], b->ev[1]));
     best = std::min(best, t);
@@ -407,8 +425,11 @@ kcl_status batch_step(kcl_batch *b, uint32_t t) {
   KCL_HIP(hipGraphLaunch(b->exec, st));
   KCL_HIP(hipEventRecord(b->ev[1], st));
   unsigned long long sv[2] = {0, 0};
+  std::vector<uint32_t> status;
   KCL_HIP(hipMemcpyAsync(sv, b->stats.p, 16, hipMemcpyDeviceToHost, st));
+  KCL_TRY(enqueue_status(b, status, st));
   KCL_HIP(hipStreamSynchronize(st));
+  KCL_TRY(check_status(status));
   float ms = 0;
   KCL_HIP(hipEventElapsedTime(&ms, b->ev[0], b->ev[1]));
   b->last_ms = ms;
@@ -480,8 +501,11 @@ kcl_status batch_densest(kcl_batch *b, kcl_densest_info *out) {
   KCL_HIP(hipMemcpyAsync(tent.data(), b->tent.p, N * 8, hipMemcpyDeviceToHost, st));
   KCL_HIP(hipMemcpyAsync(reo.data(), b->reo_local.p, N * 4, hipMemcpyDeviceToHost, st));
   KCL_HIP(hipMemcpyAsync(rho.data(), b->rho.p, N * 8, hipMemcpyDeviceToHost, st));
+  std::vector<uint32_t> status;
   KCL_HIP(hipMemcpyAsync(sv, b->stats.p, 16, hipMemcpyDeviceToHost, st));
+  KCL_TRY(enqueue_status(b, status, st));
   KCL_HIP(hipStreamSynchronize(st));
+  KCL_TRY(check_status(status));
   if (sv[1]) return fail(KCL_E_OVERFLOW, "a level received >= 2^53 cliques");
   std::vector<uint32_t> nb(count, 0);
   for (uint32_t i = 0; i < count; ++i) {

@@ -237,6 +237,7 @@ kcl_status alive_degrees(const kcl_graph *g, uint32_t k, const std::vector<uint8
                          sub->stream);
       if (e == hipSuccess) e = hipStreamSynchronize(sub->stream);
       if (e != hipSuccess) st = fail(KCL_E_HIP, "hipMemcpy: %s", hipGetErrorString(e));
+      else st = pass_status(dp, sub->stream);
     } while (0);
   }
   kcl_graph_destroy(sub);

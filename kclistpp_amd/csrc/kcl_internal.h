@@ -99,6 +99,11 @@ __host__ __device__ inline uint32_t task_edge_node(uint32_t part_np) {
   return part_np & ~kEdgeTaskFlag;
 }
 
+// Status bits a pass sets in its plan's status word (checked by the host
+// after the pass with pass_status).
+constexpr uint32_t kStatusEdgeOverflow = 1u;  // an edge task had more common
+                                              // members than its class holds
+
 enum PassMode : uint32_t {
   kCreditWinner = 0,  // each clique credits its minimum-key member
   kCreditRoot = 1,    // each clique credits its root (per-root clique counts)
@@ -148,6 +153,31 @@ kcl_status device_props(DeviceProps *out);
 kcl_status launch_pass(const kcl_graph *g, const uint64_t *d_key, uint32_t k, PassMode mode,
                        DevicePlan &plan, unsigned long long *d_cnt, hipStream_t stream,
                        unsigned long long *prof = nullptr);
+// A pass over task and ticket buffers the caller owns: tasks in the layout
+// of DevicePlan (vertex tasks by class, then edge tasks), tickets of
+// kPassTicketWords u32 (the last one is the status word).  The begin/count
+// arrays are host arrays of kNumClasses (+1 for begins) entries.
+constexpr uint32_t kPassTicketWords = 2 * 2 * kNumClasses + 1;
+struct PassLaunch {
+  const Task *tasks = nullptr;
+  uint32_t *tickets = nullptr;
+  const uint32_t *class_begin = nullptr;
+  const uint32_t *class_count = nullptr;
+  const uint32_t *eclass_begin = nullptr;
+  const uint32_t *eclass_count = nullptr;
+  const uint32_t *light_from = nullptr;
+  uint32_t max_per_cu = 0;
+};
+kcl_status launch_pass_raw(const kcl_graph *g, const uint64_t *d_key, uint32_t k, PassMode mode,
+                           const PassLaunch &plan, unsigned long long *d_cnt, hipStream_t stream,
+                           unsigned long long *prof = nullptr);
+// Device address of the plan's status word (zeroed at the start of every
+// pass, set by the kernels; kStatus* bits).
+uint32_t *pass_status_word(DevicePlan &plan);
+// Error for a status word read back after a pass (KCL_OK when 0).
+kcl_status pass_status_error(uint32_t status);
+// Reads the status word of the plan's last pass on `stream` (synchronous).
+kcl_status pass_status(DevicePlan &plan, hipStream_t stream);
 
 // Largest core bitmap side (ids below it get a dense adjacency bitmap).
 constexpr uint32_t kCoreMax = 16384;

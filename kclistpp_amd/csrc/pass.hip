@@ -51,11 +51,14 @@ uint32_t blocks_per_cu(uint32_t wclass, const DeviceProps &props, uint32_t waves
 
 namespace {
 
-// Two counters per class (single claims, chunked claims), vertex and edge.
+// Two counters per class (single claims, chunked claims), vertex and edge,
+// then the pass's status word.
 constexpr uint32_t kTickets = 2 * 2 * kNumClasses;
+constexpr uint32_t kTicketWords = kPassTicketWords;
+static_assert(kTicketWords == kTickets + 1, "tickets, then the status word");
 
 __global__ void zero_tickets_kernel(uint32_t *t) {
-  if (threadIdx.x < kTickets) t[threadIdx.x] = 0;
+  if (threadIdx.x < kTicketWords) t[threadIdx.x] = 0;
 }
 
 }  // namespace
@@ -69,11 +72,26 @@ uint32_t class_slots(uint32_t wclass, uint32_t k) {
 kcl_status launch_pass(const kcl_graph *g, const uint64_t *d_key, uint32_t k, PassMode mode,
                        DevicePlan &plan, unsigned long long *d_cnt, hipStream_t stream,
                        unsigned long long *prof) {
+  PassLaunch pl;
+  pl.tasks = plan.tasks.p;
+  pl.tickets = plan.tickets.p;
+  pl.class_begin = plan.class_begin;
+  pl.class_count = plan.class_count;
+  pl.eclass_begin = plan.eclass_begin;
+  pl.eclass_count = plan.eclass_count;
+  pl.light_from = plan.light_from;
+  pl.max_per_cu = plan.max_per_cu;
+  return launch_pass_raw(g, d_key, k, mode, pl, d_cnt, stream, prof);
+}
+
+kcl_status launch_pass_raw(const kcl_graph *g, const uint64_t *d_key, uint32_t k, PassMode mode,
+                           const PassLaunch &plan, unsigned long long *d_cnt, hipStream_t stream,
+                           unsigned long long *prof) {
   DeviceProps props;
   KCL_TRY(device_props(&props));
   // A kernel rather than a memset node, so the pass can be graph-captured.
-  static_assert(kTickets <= 64, "tickets of one pass fit one wavefront");
-  zero_tickets_kernel<<<1, 64, 0, stream>>>(plan.tickets.p);
+  static_assert(kTicketWords <= 64, "tickets of one pass fit one wavefront");
+  zero_tickets_kernel<<<1, 64, 0, stream>>>(plan.tickets);
   KCL_HIP(hipGetLastError());
   // The class launches go round-robin to the pass's stream and the graph's
   // side streams, so a class's tail overlaps the next class (credits are
@@ -103,9 +121,10 @@ kcl_status launch_pass(const kcl_graph *g, const uint64_t *d_key, uint32_t k, Pa
                          q ? g->side[q - 1] : stream,
                          (const uint32_t *)g->d_cd.p, (const uint32_t *)g->d_adj.p, d_key,
                          (const unsigned long long *)(g->core_h ? g->d_core.p : nullptr),
-                         g->core_h, plan.tasks.p + begin[c], count,
+                         g->core_h, plan.tasks + begin[c], count,
                          grp ? count : std::min(plan.light_from[c], count), g->dense_items,
-                         plan.tickets.p + 2 * (grp * kNumClasses + c), (uint32_t)mode, d_cnt,
+                         plan.tickets + 2 * (grp * kNumClasses + c), plan.tickets + kTickets,
+                         (uint32_t)mode, d_cnt,
                          prof ? prof + 2 * (size_t)begin[c] : nullptr);
       KCL_HIP(hipGetLastError());
     }
@@ -115,6 +134,26 @@ kcl_status launch_pass(const kcl_graph *g, const uint64_t *d_key, uint32_t k, Pa
     KCL_HIP(hipStreamWaitEvent(stream, g->join[i], 0));
   }
   return KCL_OK;
+}
+
+uint32_t *pass_status_word(DevicePlan &plan) { return plan.tickets.p + kTickets; }
+
+kcl_status pass_status_error(uint32_t status) {
+  if (status & kStatusEdgeOverflow)
+    return fail(KCL_E_UNSUPPORTED,
+                "an edge task had more common out-neighbours than its word class holds "
+                "(the plan's member counts disagree with the graph); the pass's counts are "
+                "incomplete");
+  if (status) return fail(KCL_E_HIP, "enumeration pass status 0x%x", status);
+  return KCL_OK;
+}
+
+kcl_status pass_status(DevicePlan &plan, hipStream_t stream) {
+  if (!plan.tickets.p) return KCL_OK;
+  uint32_t w = 0;
+  KCL_HIP(hipMemcpyAsync(&w, pass_status_word(plan), 4, hipMemcpyDeviceToHost, stream));
+  KCL_HIP(hipStreamSynchronize(stream));
+  return pass_status_error(w);
 }
 
 namespace {
@@ -335,7 +374,7 @@ kcl_status upload_plan(const PassPlan &plan, DevicePlan *dev) {
   dev->host_tasks.insert(dev->host_tasks.end(), plan.edge_tasks.begin(), plan.edge_tasks.end());
   const uint32_t nv = (uint32_t)plan.tasks.size();
   KCL_TRY(dev->tasks.alloc(dev->host_tasks.size()));
-  KCL_TRY(dev->tickets.alloc(kTickets));
+  KCL_TRY(dev->tickets.alloc(kTicketWords));
   if (!dev->host_tasks.empty())
     KCL_HIP(hipMemcpy(dev->tasks.p, dev->host_tasks.data(), dev->host_tasks.size() * sizeof(Task),
                       hipMemcpyHostToDevice));

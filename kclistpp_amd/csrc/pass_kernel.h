@@ -271,6 +271,7 @@ __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restric
                                                        uint32_t ntasks, uint32_t light_from,
                                                        float dense_items,
                                                        uint32_t *__restrict__ ticket,
+                                                       uint32_t *__restrict__ status,
                                                        uint32_t mode,
                                                        unsigned long long *__restrict__ cnt,
                                                        unsigned long long *__restrict__ prof) {
@@ -397,6 +398,11 @@ __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restric
         }
         __syncthreads();
       }
+      // The plan put this task in a class that holds its common members
+      // (edge_members_kernel); a larger intersection means the plan and the
+      // graph disagree.  Count what fits, and flag the pass so the host
+      // turns it into an error instead of returning wrong counts.
+      if (tid == 0 && sh.ecount > (uint32_t)Lds<W>::D) atomicOr(status, kStatusEdgeOverflow);
       d = min(sh.ecount, (uint32_t)Lds<W>::D);
       for (uint32_t i = tid; i < d; i += NT) {
         const uint32_t x = sh.node[i];
@@ -820,7 +826,8 @@ __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restric
 
 using PassFn = void (*)(const uint32_t *, const uint32_t *, const uint64_t *,
                         const unsigned long long *, uint32_t, const Task *, uint32_t, uint32_t,
-                        float, uint32_t *, uint32_t, unsigned long long *, unsigned long long *);
+                        float, uint32_t *, uint32_t *, uint32_t, unsigned long long *,
+                        unsigned long long *);
 
 constexpr int kMaxTemplK = 10;  // k handled by compiled kernels: 2..kMaxTemplK
 

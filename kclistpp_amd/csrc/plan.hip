@@ -40,8 +40,7 @@ kcl_status root_counts(kcl_graph *g, uint32_t k, const PassPlan &plan, hipStream
   KCL_TRY(launch_pass(g, (const uint64_t *)key.p, k, kCreditRoot, dp, c.p, st));
   KCL_HIP(hipMemcpyAsync(out->data(), c.p, (size_t)g->n * sizeof(uint64_t),
                          hipMemcpyDeviceToHost, st));
-  KCL_HIP(hipStreamSynchronize(st));
-  return KCL_OK;
+  return pass_status(dp, st);
 }
 
 // Work per root: the steps of its k-pass (one per (k-1)-clique with that

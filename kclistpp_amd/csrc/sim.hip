@@ -219,9 +219,14 @@ kcl_status sim_step(kcl_sim *s, uint32_t t) {
   }
   KCL_HIP(hipEventRecord(s->ev[2], s->stream));
   unsigned long long st[2] = {0, 0};
+  uint32_t status = 0;
   KCL_HIP(hipMemcpyAsync(st, s->stats.p, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                          s->stream));
+  if (!s->trivial)
+    KCL_HIP(hipMemcpyAsync(&status, pass_status_word(s->plan), 4, hipMemcpyDeviceToHost,
+                           s->stream));
   KCL_HIP(hipStreamSynchronize(s->stream));
+  KCL_TRY(pass_status_error(status));
   float k_ms = 0, p_ms = 0;
   KCL_HIP(hipEventElapsedTime(&k_ms, s->ev[0], s->ev[1]));
   KCL_HIP(hipEventElapsedTime(&p_ms, s->ev[0], s->ev[2]));
@@ -276,11 +281,15 @@ kcl_status sim_level_pass(kcl_sim *s, bool full_key, std::vector<unsigned long l
   reo.resize(n);
   rho.resize(n);
   unsigned long long stv[2] = {0, 0};
+  uint32_t status = 0;
+  if (!s->trivial)
+    KCL_HIP(hipMemcpyAsync(&status, pass_status_word(s->plan), 4, hipMemcpyDeviceToHost, st));
   KCL_HIP(hipMemcpyAsync(tent.data(), s->tent.p, (size_t)n * 8, hipMemcpyDeviceToHost, st));
   KCL_HIP(hipMemcpyAsync(reo.data(), s->reordered.p, (size_t)n * 4, hipMemcpyDeviceToHost, st));
   KCL_HIP(hipMemcpyAsync(rho.data(), s->rho.p, (size_t)n * 8, hipMemcpyDeviceToHost, st));
   KCL_HIP(hipMemcpyAsync(stv, s->stats.p, 16, hipMemcpyDeviceToHost, st));
   KCL_HIP(hipStreamSynchronize(st));
+  KCL_TRY(pass_status_error(status));
   if (stv[1]) return fail(KCL_E_OVERFLOW, "a level received >= 2^53 cliques");
   return KCL_OK;
 }
@@ -469,7 +478,7 @@ extern "C" kcl_status kcl_sim_profile_pass(kcl_sim *s, uint32_t *ntasks, uint32_
   std::vector<unsigned long long> h(2 * (size_t)nt);
   KCL_HIP(hipMemcpyAsync(h.data(), prof.p, h.size() * 8, hipMemcpyDeviceToHost, s->stream));
   KCL_TRY(kcl::zero_u64(s->cnt.p, s->g->n, s->stream));  // the pass leaves no credits behind
-  KCL_HIP(hipStreamSynchronize(s->stream));
+  KCL_TRY(kcl::pass_status(s->plan, s->stream));
   for (uint32_t i = 0; i < nt; ++i) {
     const kcl::Task &t = s->plan.host_tasks[i];
     if (root) root[i] = t.s;

@@ -242,6 +242,7 @@ kcl_status trim_run(kcl_trim *T, const uint32_t *pairs, uint64_t m, uint32_t n_i
       e = hipMemcpyAsync(T->nck.data(), cnt.p, (size_t)g->n * 8, hipMemcpyDeviceToHost, g->stream);
       if (e == hipSuccess) e = hipStreamSynchronize(g->stream);
       if (e != hipSuccess) st = fail(KCL_E_HIP, "hipMemcpy: %s", hipGetErrorString(e));
+      else st = pass_status(dp, g->stream);
     } while (0);
   }  // else no k-clique can exist: the degrees stay zero
   std::vector<uint32_t> cd = g->cd, adj = g->adj;  // DAG rows, ascending

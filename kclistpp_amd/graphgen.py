@@ -23,6 +23,7 @@ __all__ = [
     "erdos_renyi",
     "chung_lu",
     "rmat",
+    "bipartite_plus",
     "plant_clique",
     "plant_community",
     "finalize",
@@ -144,6 +145,24 @@ def rmat(scale: int, m: int, a: float, b: float, c: float, seed: int):
     pairs = _keys_to_pairs(_sample_unique(rng, n, m, draw), n)
     perm = rng.permutation(n).astype(np.uint32)
     return perm[pairs], rng
+
+
+def bipartite_plus(part: int, p_in: float, seed: int) -> np.ndarray:
+    """Complete bipartite K_{part,part} plus each pair inside a side with
+    probability ``p_in``, finalized: DAG out-degrees reach about ``part``
+    (wide bitmap rows) while the cliques stay few (a clique takes at most a
+    clique of each side)."""
+    rng = np.random.default_rng(seed)
+    a = np.arange(part)
+    b = np.arange(part, 2 * part)
+    cross = np.stack(np.meshgrid(a, b, indexing="ij"), -1).reshape(-1, 2)
+    intra = []
+    for side in (a, b):
+        iu, ju = np.triu_indices(part, 1)
+        keep = rng.random(iu.size) < p_in
+        intra.append(np.stack([side[iu[keep]], side[ju[keep]]], 1))
+    pairs = np.concatenate([cross, *intra]).astype(np.uint32)
+    return finalize(pairs, rng)
 
 
 def plant_clique(pairs: np.ndarray, n: int, size: int, rng: np.random.Generator) -> np.ndarray:

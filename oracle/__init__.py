@@ -221,6 +221,7 @@ class RefHarness:
         L.kclref_report.argtypes = [C.c_void_p, _u32p, _f64p, C.c_void_p, C.c_void_p,
                                     C.c_void_p]
         L.kclref_set_rho.argtypes = [C.c_void_p, _f64p]
+        L.kclref_pava_level.argtypes = [C.c_void_p, _u32p, _f64p]
         L.kclref_time_fw_roots.argtypes = [C.c_void_p, _u32p, C.c_uint, C.POINTER(C.c_double)]
         L.kclref_time_fw_roots.restype = C.c_double
         L.kclref_free.argtypes = [C.c_void_p]
@@ -289,6 +290,14 @@ class RefHarness:
 
     def set_rho(self, h, rho):
         self.L.kclref_set_rho(h, np.ascontiguousarray(rho, np.float64))
+
+    def pava_level(self, h, level: np.ndarray) -> np.ndarray:
+        """One PAVA-preprocess pass with the given level vector; returns
+        rho_tentative (indexed by level)."""
+        n = self.L.kclref_n(h)
+        out = np.zeros(max(n, 1), np.float64)
+        self.L.kclref_pava_level(h, np.ascontiguousarray(level, np.uint32), out)
+        return out[:n]
 
     def time_fw_roots(self, h, roots: np.ndarray):
         v = C.c_double(0)

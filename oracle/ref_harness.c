@@ -146,6 +146,18 @@ void kclref_report(void *p, unsigned *nm, double *dens_ub, unsigned *level_out,
   if (tent_out) memcpy(tent_out, rho_tentative, g->n * sizeof(double));
 }
 
+/* One CDF_CliqueEnum(PAVA_PREPROCESS) with the caller's level vector
+ * (kCDensestCmpSync.c:217-246 read the global level[]); copies
+ * rho_tentative (indexed by level) to tent_out. */
+void kclref_pava_level(void *p, const unsigned *level_in, double *tent_out) {
+  kclref *h = (kclref *)p;
+  Graph *g = h->g;
+  memcpy(level, level_in, g->n * sizeof(unsigned));
+  for (unsigned i = 0; i < g->n; ++i) rho_tentative[i] = 0;
+  CDF_CliqueEnum(g, (unsigned char)h->k, REF_PAVA_PREPROCESS);
+  memcpy(tent_out, rho_tentative, g->n * sizeof(double));
+}
+
 /* Set rho from the caller (to time or check a pass at a given state). */
 void kclref_set_rho(void *p, const double *src) {
   kclref *h = (kclref *)p;

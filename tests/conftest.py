@@ -26,6 +26,19 @@ def load_golden(case: str) -> dict:
         return {k: z[k] for k in z.files}
 
 
+def golden_pairs(gd: dict) -> np.ndarray:
+    """The input edges of a golden case: stored, or regenerated from the
+    seeded generator the case names (checked against its SHA-256)."""
+    if "pairs" in gd:
+        return gd["pairs"]
+    import hashlib
+    from kclistpp_amd import graphgen as gg
+    part, seed = (int(x) for x in gd["gen"])
+    pairs = gg.bipartite_plus(part, float(gd["p_in"]), seed)
+    assert hashlib.sha256(pairs.tobytes()).hexdigest() == str(gd["pairs_sha"][0])
+    return pairs
+
+
 @pytest.fixture(scope="session")
 def oracle():
     from oracle import Oracle

@@ -17,7 +17,13 @@ what the reference computed:
 and for trimming cases the outputs of the reference kClistCoreTrim program
 (ckdeg, ckcore, ckdens, trimmed edges, stat line).
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [small] [wide] [big] [batch]
+    python tests/golden/make_golden.py --big-case <case> ...   # one workload case
+
+Groups: small (hand-made and small seeded graphs, full vectors), wide
+(wide bitmap rows at k = 6-8 and a wide trimming case), big (the benchmark
+workloads and their scaled families, as digests), batch (the 512-graph mixed
+batch, per-graph digests).  Existing index entries are kept.
 """
 
 from __future__ import annotations
@@ -88,12 +94,41 @@ SIM_CASES = [
 
 TRIM_CASES = [("cl3k", 4), ("cl3k", 5), ("cl1k_dense", 5), ("er2k_k12", 4), ("k7_k5_bridge", 4)]
 
-# Full-size workload cases: vectors stored as digests.
+# Workload cases: vectors stored as digests.  One per benchmark config at
+# the largest size the reference finishes here in minutes: the k = 4 case
+# and the k = 5 case at full size, the k = 6 exact case's Chung-Lu family
+# (gamma 2.1 plus the 200-vertex near-clique with p = 0.9, 1.7e10 6-cliques)
+# at 1/256 of its nodes and edges, and the R-MAT case at scales 10 and 11
+# (k = 7 and 8; 2e9-1.2e10 cliques per pass).
 BIG_CASES = [
     # (workload name, seed, scale_down, k, num_iter)
     ("er20k", 1, 1, 4, 10),
     ("cl1m", 3, 64, 5, 4),
+    ("cl1m", 1, 1, 5, 4),
+    ("cl4m", 1, 256, 6, 2),
+    ("rmat23", 1, 8192, 7, 2),
+    ("rmat23", 1, 8192, 8, 1),
+    ("rmat23", 1, 4096, 7, 1),
 ]
+
+# Wide out-neighbourhoods: K_{part,part} plus sparse edges inside each side,
+# so that roots fill the 6-, 8-, 12- and 16-word bitmap classes; one
+# Frank-Wolfe pass at a seeded rho with many ties and one PAVA pass at a
+# seeded level permutation per k.
+WIDE_CASES = [
+    # (part, p_in, seed, ks)
+    (340, 0.06, 340, (6, 7, 8)),
+    (480, 0.05, 480, (6, 7, 8)),
+    (700, 0.04, 700, (6, 7, 8)),
+    (940, 0.035, 940, (6, 7, 8)),
+]
+
+# Trimming of a wide graph (k-clique degrees over rows of 12 words).
+WIDE_TRIM_CASES = [(700, 0.04, 700, 6)]
+
+# The mixed batch: every graph of graphgen.batch_specs(512, 1000) with its
+# own k, num_iter = 4 (iterations 1-4, reports at T = 1, 2, 4).
+BATCH_T = 4
 
 
 def run_sim_case(ref: RefHarness, path: str, k: int, num_iter: int, store_vectors: bool):
@@ -164,6 +199,144 @@ def run_trim_case(workdir: str, path: str, k: int, tag: str):
     }
 
 
+def run_wide_case(ref: RefHarness, pairs: np.ndarray, k: int, seed: int) -> dict:
+    h = ref.load_pairs(pairs)
+    info = ref.info(h)
+    ref.start(h, k)
+    rng = np.random.default_rng(seed * 10 + k)
+    rho = rng.integers(0, 3, info["n"]).astype(np.float64)
+    level = rng.permutation(info["n"]).astype(np.uint32)
+    ref.set_rho(h, rho)
+    add, _ = ref.step(h, 1)  # the pass runs at the rho just set
+    tent = ref.pava_level(h, level)
+    ref.free(h)
+    fw = add.astype(np.uint64)
+    assert (fw.astype(np.float64) == add).all() and (tent.astype(np.uint64) == tent).all()
+    return {"n": np.uint32(info["n"]), "core": np.uint32(info["core"]), "k": np.uint32(k),
+            "rho": rho, "level": level, "fw": fw, "visits": np.uint64(fw.sum()),
+            "tent": tent.astype(np.uint64)}
+
+
+def one_wide_case(part: int, p_in: float, seed: int, k: int, out: str) -> None:
+    ref = RefHarness()
+    ref.threads(8)
+    pairs = gg.bipartite_plus(part, p_in, seed)
+    data = run_wide_case(ref, pairs, k, seed)
+    data["pairs_sha"] = np.array([sha(pairs)])
+    data["gen"] = np.array([part, seed], np.uint32)
+    data["p_in"] = np.float64(p_in)
+    np.savez(out, **data)
+
+
+def one_batch_graph(i: int, T: int, out: str) -> None:
+    """Child process: graph i of the mixed batch through the reference's own
+    load (SortByCore/Relabel/MakeGraph), T iterations and the reports."""
+    spec = list(gg.batch_specs(512, first_seed=1000))[i]
+    pairs = gg.batch_graph(spec)
+    ref = RefHarness()
+    ref.threads(2)
+    h = ref.load_pairs(pairs)
+    info = ref.info(h)
+    rank, e = ref.rank(h), ref.edges(h)
+    ref.start(h, spec["k"])
+    shas, recs = [], []
+    t, Tp = 1, 1
+    rho = np.zeros(info["n"])
+    while Tp <= T:
+        while t <= Tp:
+            add, rho = ref.step(h, t)
+            c = add.astype(np.uint64)
+            assert (c.astype(np.float64) == add).all()
+            shas.append(sha(c))
+            t += 1
+        rec, _, _, _ = ref.report(h)
+        recs.append([Tp, rec["n"], rec["m"], int(np.float64(rec["density"]).view(np.uint64)),
+                     int(np.float64(rec["ub"]).view(np.uint64))])
+        Tp <<= 1
+    ref.free(h)
+    np.savez(out, n=np.uint32(info["n"]), core=np.uint32(info["core"]),
+             pairs_sha=np.array([sha(pairs)]), rank_sha=np.array([sha(rank)]),
+             edges_sha=np.array([sha(e)]), count_sha=np.array(shas),
+             rho_sha=np.array([sha(rho.view(np.uint64))]), records=np.array(recs, np.uint64))
+
+
+def _child(args: list[str]) -> dict:
+    with tempfile.TemporaryDirectory() as td:
+        out = os.path.join(td, "case.npz")
+        subprocess.run([sys.executable, os.path.abspath(__file__), *args, out], check=True,
+                       capture_output=True)
+        with np.load(out, allow_pickle=False) as z:
+            return {key: z[key] for key in z.files}
+
+
+def make_batch(workers: int = 4) -> str:
+    import concurrent.futures as cf
+    count = 512
+    with cf.ThreadPoolExecutor(max_workers=workers) as ex:
+        parts = list(ex.map(lambda i: _child(["--one-batch", str(i), str(BATCH_T)]),
+                            range(count)))
+    data = {key: np.stack([p[key] for p in parts]) for key in parts[0]}
+    data["num_iter"] = np.uint32(BATCH_T)
+    case = f"batch512_T{BATCH_T}"
+    np.savez_compressed(os.path.join(HERE, case + ".npz"), **data)
+    print(case, "graphs", count, flush=True)
+    return case
+
+
+def make_wide() -> list[str]:
+    cases = []
+    for part, p_in, seed, ks in WIDE_CASES:
+        for k in ks:
+            case = f"wide_bip{part}_k{k}"
+            data = _child(["--one-wide", str(part), repr(p_in), str(seed), str(k)])
+            np.savez_compressed(os.path.join(HERE, case + ".npz"), **data)
+            cases.append(case)
+            print(case, "core", int(data["core"]), "visits", int(data["visits"]), flush=True)
+    return cases
+
+
+def make_wide_trim() -> list[str]:
+    cases = []
+    with tempfile.TemporaryDirectory() as td:
+        for part, p_in, seed, k in WIDE_TRIM_CASES:
+            name = f"bip{part}"
+            pairs = gg.bipartite_plus(part, p_in, seed)
+            path = os.path.join(td, name + ".txt")
+            gg.write_text(pairs, path)
+            data = run_trim_case(td, path, k, name)
+            data["pairs_sha"] = np.array([sha(pairs)])
+            data["gen"] = np.array([part, seed], np.uint32)
+            data["p_in"] = np.float64(p_in)
+            data["k"] = np.uint32(k)
+            case = f"trim_{name}_k{k}"
+            np.savez_compressed(os.path.join(HERE, case + ".npz"), **data)
+            cases.append(case)
+            print(case, "stat", data["stat"].tolist(), flush=True)
+    return cases
+
+
+def make_big(only=None) -> list[str]:
+    cases = []
+    with tempfile.TemporaryDirectory() as td:
+        for wname, seed, sd, k, T in BIG_CASES:
+            case = f"big_{wname}_s{seed}_d{sd}_k{k}_T{T}"
+            if only and case not in only:
+                continue
+            pairs = gg.workload(wname, seed=seed, scale_down=sd)
+            p = os.path.join(td, case + ".txt")
+            gg.write_text(pairs, p)
+            data = sim_case_in_child(p, k, T, store_vectors=False)
+            os.remove(p)
+            data["pairs_sha"] = np.array([sha(pairs)])
+            data["workload"] = np.array([wname])
+            data["seed"] = np.uint32(seed)
+            data["scale_down"] = np.uint32(sd)
+            np.savez_compressed(os.path.join(HERE, case + ".npz"), **data)
+            cases.append(case)
+            print(case, "records", data["records"][:, :3].tolist(), flush=True)
+    return cases
+
+
 def one_sim_case(path: str, k: int, T: int, store_vectors: bool, out: str) -> None:
     """Child process: the reference keeps per-thread scratch sized by the
     first graph it sees (kCDensestCmpSync.c:74-80), so each graph gets a
@@ -183,52 +356,64 @@ def sim_case_in_child(path: str, k: int, T: int, store_vectors: bool) -> dict:
             return {key: z[key] for key in z.files}
 
 
-def main():
+def _update_index(group: str, cases: list[str]) -> None:
+    path = os.path.join(HERE, "index.json")
+    index = json.load(open(path)) if os.path.exists(path) else {}
+    have = index.get(group, [])
+    index[group] = have + [c for c in cases if c not in have]
+    with open(path, "w") as f:
+        json.dump(index, f, indent=1)
+
+
+def main(groups: list[str]):
     subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "all", "ref"], check=True,
                    capture_output=True)
-    graphs = dict(small_graphs())
-    index = {"sim": [], "trim": [], "big": []}
-    with tempfile.TemporaryDirectory() as td:
-        paths = {}
-        for name, pairs in graphs.items():
-            p = os.path.join(td, name + ".txt")
-            gg.write_text(pairs, p)
-            paths[name] = p
-        for name, k, T in SIM_CASES:
-            case = f"sim_{name}_k{k}_T{T}"
-            data = sim_case_in_child(paths[name], k, T, store_vectors=True)
-            data["pairs"] = graphs[name]
-            np.savez_compressed(os.path.join(HERE, case + ".npz"), **data)
-            index["sim"].append(case)
-            print(case, "records", data["records"][:, :3].tolist(), flush=True)
-        for name, k in TRIM_CASES:
-            case = f"trim_{name}_k{k}"
-            data = run_trim_case(td, paths[name], k, name)
-            data["pairs"] = graphs[name]
-            data["k"] = np.uint32(k)
-            np.savez_compressed(os.path.join(HERE, case + ".npz"), **data)
-            index["trim"].append(case)
-            print(case, "stat", data["stat"].tolist(), flush=True)
-        for wname, seed, sd, k, T in BIG_CASES:
-            case = f"big_{wname}_s{seed}_d{sd}_k{k}_T{T}"
-            pairs = gg.workload(wname, seed=seed, scale_down=sd)
-            p = os.path.join(td, case + ".txt")
-            gg.write_text(pairs, p)
-            data = sim_case_in_child(p, k, T, store_vectors=False)
-            data["pairs_sha"] = np.array([sha(pairs)])
-            data["workload"] = np.array([wname])
-            data["seed"] = np.uint32(seed)
-            data["scale_down"] = np.uint32(sd)
-            np.savez_compressed(os.path.join(HERE, case + ".npz"), **data)
-            index["big"].append(case)
-            print(case, "records", data["records"][:, :3].tolist(), flush=True)
-    with open(os.path.join(HERE, "index.json"), "w") as f:
-        json.dump(index, f, indent=1)
+    if "small" in groups:
+        graphs = dict(small_graphs())
+        index = {"sim": [], "trim": []}
+        with tempfile.TemporaryDirectory() as td:
+            paths = {}
+            for name, pairs in graphs.items():
+                p = os.path.join(td, name + ".txt")
+                gg.write_text(pairs, p)
+                paths[name] = p
+            for name, k, T in SIM_CASES:
+                case = f"sim_{name}_k{k}_T{T}"
+                data = sim_case_in_child(paths[name], k, T, store_vectors=True)
+                data["pairs"] = graphs[name]
+                np.savez_compressed(os.path.join(HERE, case + ".npz"), **data)
+                index["sim"].append(case)
+                print(case, "records", data["records"][:, :3].tolist(), flush=True)
+            for name, k in TRIM_CASES:
+                case = f"trim_{name}_k{k}"
+                data = run_trim_case(td, paths[name], k, name)
+                data["pairs"] = graphs[name]
+                data["k"] = np.uint32(k)
+                np.savez_compressed(os.path.join(HERE, case + ".npz"), **data)
+                index["trim"].append(case)
+                print(case, "stat", data["stat"].tolist(), flush=True)
+        _update_index("sim", index["sim"])
+        _update_index("trim", index["trim"])
+    if "wide" in groups:
+        _update_index("wide", make_wide())
+        _update_index("trim", make_wide_trim())
+    if "big" in groups:
+        _update_index("big", make_big())
+    if "batch" in groups:
+        _update_index("batch", [make_batch()])
 
 
 if __name__ == "__main__":
     if len(sys.argv) > 1 and sys.argv[1] == "--one":
         a = sys.argv[2:]
         one_sim_case(a[0], int(a[1]), int(a[2]), a[3] == "1", a[4])
+    elif len(sys.argv) > 1 and sys.argv[1] == "--one-wide":
+        a = sys.argv[2:]
+        one_wide_case(int(a[0]), float(a[1]), int(a[2]), int(a[3]), a[4])
+    elif len(sys.argv) > 1 and sys.argv[1] == "--one-batch":
+        a = sys.argv[2:]
+        one_batch_graph(int(a[0]), int(a[1]), a[2])
+    elif len(sys.argv) > 1 and sys.argv[1] == "--big-case":
+        _update_index("big", make_big(set(sys.argv[2:])))
     else:
-        main()
+        main(sys.argv[1:] or ["small", "wide", "big", "batch"])

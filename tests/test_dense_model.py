@@ -84,16 +84,7 @@ def dense_model_counts(g, k, rho, skip=0, win_extra=0):
 
 
 def _bipartite_plus(part, p_in, seed):
-    rng = np.random.default_rng(seed)
-    a = np.arange(part)
-    b = np.arange(part, 2 * part)
-    cross = np.stack(np.meshgrid(a, b, indexing="ij"), -1).reshape(-1, 2)
-    intra = []
-    for side in (a, b):
-        iu, ju = np.triu_indices(part, 1)
-        keep = rng.random(iu.size) < p_in
-        intra.append(np.stack([side[iu[keep]], side[ju[keep]]], 1))
-    return gg.finalize(np.concatenate([cross, *intra]).astype(np.uint32), rng)
+    return gg.bipartite_plus(part, p_in, seed)
 
 
 def _community(seed):

@@ -147,3 +147,62 @@ def test_batch_rejects_bad_input(kcl, small):
         kcl.Batch(graphs[:2], ks[:1])
     with pytest.raises(kcl.KclError):
         kcl.Batch(graphs[:1], [1])
+
+
+def _batch_graph_of(i):
+    spec = list(gg.batch_specs(512, first_seed=1000))[i]
+    return gg.batch_graph(spec)
+
+
+@pytest.mark.timeout(900)
+def test_batch512_matches_reference(kcl):
+    """The mixed batch as benchmarked: all 512 Chung-Lu graphs (n up to
+    2e5, k = 3..7) in one batch, against digests of the reference's own run
+    of every graph: ranks, DAG edges, the count vector of iterations 1-4,
+    the reports at T = 1, 2, 4 and the final rho bits."""
+    import concurrent.futures as cf
+    import hashlib
+    import multiprocessing as mp
+    from conftest import golden_index, load_golden
+
+    if "batch" not in golden_index():
+        pytest.skip("no batch golden case")
+    gd = load_golden(golden_index()["batch"][0])
+    T = int(gd["num_iter"])
+    specs = list(gg.batch_specs(512, first_seed=1000))
+    sha = lambda a: hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()  # noqa: E731
+    _say("batch512: generating 512 graphs")
+    with cf.ProcessPoolExecutor(8, mp_context=mp.get_context("spawn")) as ex:
+        pairs = list(ex.map(_batch_graph_of, range(512), chunksize=8))
+    graphs = []
+    try:
+        for i, p in enumerate(pairs):
+            assert sha(p) == str(gd["pairs_sha"][i][0]), f"graph {i} input differs"
+            g = kcl.Graph(p)
+            graphs.append(g)
+            assert sha(g.rank()) == str(gd["rank_sha"][i][0]), f"graph {i} ranks differ"
+            assert sha(g.edges()) == str(gd["edges_sha"][i][0]), f"graph {i} DAG differs"
+        del pairs
+        ks = [sp["k"] for sp in specs]
+        _say("batch512: building the batch")
+        with kcl.Batch(graphs, ks, tune=True) as b:
+            t, Tp, rep = 1, 1, 0
+            while Tp <= T:
+                while t <= Tp:
+                    b.step(t)
+                    for i in range(512):
+                        assert sha(b.counts(i)) == str(gd["count_sha"][i][t - 1]), \
+                            f"graph {i} (k={ks[i]}) counts differ at t={t}"
+                    t += 1
+                dens = b.find_densest()
+                for i, d in enumerate(dens):
+                    got = [Tp, d.n, d.m, int(_bits(d.density)), int(_bits(d.ub))]
+                    assert got == gd["records"][i][rep].tolist(), f"graph {i} report T={Tp}"
+                rep += 1
+                Tp <<= 1
+            for i in range(512):
+                assert sha(b.rho(i).view(np.uint64)) == str(gd["rho_sha"][i][0]), \
+                    f"graph {i} rho differs"
+    finally:
+        for g in graphs:
+            g.close()

@@ -9,13 +9,14 @@ import subprocess
 import numpy as np
 import pytest
 
-from conftest import ROOT, golden_index, load_golden
+from conftest import ROOT, golden_index, golden_pairs, load_golden
 from kclistpp_amd import graphgen as gg
 
 pytestmark = pytest.mark.gpu
 
 SIM = golden_index()["sim"]
 BIG = golden_index()["big"]
+WIDE = golden_index()["wide"]
 
 
 def _bits(x):
@@ -111,31 +112,76 @@ def test_cli_drop_in_stat_file(kcl, tmp_path):
         assert f[4] == "%.12f" % np.uint64(rec[4]).view(np.float64)
 
 
+_BIG_PAIRS = {}
+
+
+def _big_pairs(gd):
+    key = (str(gd["workload"][0]), int(gd["seed"]), int(gd["scale_down"]))
+    if key not in _BIG_PAIRS:
+        _BIG_PAIRS.clear()
+        pairs = gg.workload(key[0], seed=key[1], scale_down=key[2])
+        assert hashlib.sha256(pairs.tobytes()).hexdigest() == str(gd["pairs_sha"][0])
+        _BIG_PAIRS[key] = pairs
+    return _BIG_PAIRS[key]
+
+
+def _run_digests(kcl, s, T, after_first=None):
+    shas, recs = [], []
+    t, Tp = 1, 1
+    while Tp <= T:
+        while t <= Tp:
+            s.step(t)
+            shas.append(hashlib.sha256(s.counts().tobytes()).hexdigest())
+            if t == 1 and after_first is not None:
+                after_first(s)
+            t += 1
+        d = s.find_densest()
+        recs.append([Tp, d.n, d.m, _bits(d.density), _bits(d.ub)])
+        Tp <<= 1
+    return shas, np.array(recs, np.uint64)
+
+
+# Schedules of the workload cases: counts must not depend on the plan.
+#   degree   the plain schedule (d+^2 order and splits)
+#   counted  costs from a counting pass (the default)
+#   learned  the shipped cost model
+#   online   the shipped model retrained on the graph's own first-pass
+#            timings, the plan rebuilt with it after iteration 1
+_SCHEDULES = ["degree", "counted", "learned", "online"]
+
+
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("schedule", _SCHEDULES)
 @pytest.mark.parametrize("case", BIG)
-def test_full_size_workloads_match_reference(kcl, case):
+def test_full_size_workloads_match_reference(kcl, case, schedule):
+    """The benchmark workloads (the k = 4 and k = 5 cases at full size, the
+    k = 6 exact case's Chung-Lu family with its 200-vertex near-clique, the
+    R-MAT case at scales 10 and 11 with k = 7, 8) against digests of the
+    reference's own vectors: ranks, DAG edges, every iteration's count
+    vector, the final rho bits and each report record."""
+    from kclistpp_amd import train_cost_model as tcm
     gd = load_golden(case)
-    pairs = gg.workload(str(gd["workload"][0]), seed=int(gd["seed"]),
-                        scale_down=int(gd["scale_down"]))
-    assert hashlib.sha256(pairs.tobytes()).hexdigest() == str(gd["pairs_sha"][0])
+    pairs = _big_pairs(gd)
     k, T = int(gd["k"]), int(gd["num_iter"])
-    _say("big", case)
-    with kcl.Graph(pairs) as g, kcl.Sim(g, k) as s:
+    _say("big", case, schedule)
+    model = kcl.CostModel(path=os.path.join(kcl.PKG, "cost_model.kclcm")) \
+        if schedule in ("learned", "online") else None
+    sched = kcl.SCHEDULES["learned" if schedule == "online" else schedule]
+    retrained = []
+
+    def retrain(s):
+        retrained.append(tcm.online(s, model, epochs=10))
+
+    with kcl.Graph(pairs) as g, kcl.Sim(g, k, schedule=sched, model=model) as s:
         assert hashlib.sha256(g.rank().tobytes()).hexdigest() == str(gd["rank_sha"][0])
         assert hashlib.sha256(g.edges().tobytes()).hexdigest() == str(gd["edges_sha"][0])
-        shas, recs = [], []
-        t, Tp = 1, 1
-        while Tp <= T:
-            while t <= Tp:
-                s.step(t)
-                shas.append(hashlib.sha256(s.counts().tobytes()).hexdigest())
-                t += 1
-            d = s.find_densest()
-            recs.append([Tp, d.n, d.m, _bits(d.density), _bits(d.ub)])
-            Tp <<= 1
+        shas, recs = _run_digests(kcl, s, T, retrain if schedule == "online" else None)
         assert shas == [str(x) for x in gd["count_sha"]]
-        assert np.array_equal(np.array(recs, np.uint64), gd["records"])
+        assert np.array_equal(recs, gd["records"])
         assert hashlib.sha256(s.rho().view(np.uint64).tobytes()).hexdigest() == \
             str(gd["rho_sha"][0])
+    for m in retrained + ([model] if model is not None else []):
+        m.close()
 
 
 # ---------------------------------------------------------------------------
@@ -168,17 +214,7 @@ def test_clique_enum_boundary_vs_oracle(kcl, oracle, k):
 def _bipartite_plus(part, p_in, seed):
     """Complete bipartite K_{part,part} plus sparse edges inside each side:
     out-degrees reach ~part (the widest word classes) with few cliques."""
-    rng = np.random.default_rng(seed)
-    a = np.arange(part)
-    b = np.arange(part, 2 * part)
-    cross = np.stack(np.meshgrid(a, b, indexing="ij"), -1).reshape(-1, 2)
-    intra = []
-    for side in (a, b):
-        iu, ju = np.triu_indices(part, 1)
-        keep = rng.random(iu.size) < p_in
-        intra.append(np.stack([side[iu[keep]], side[ju[keep]]], 1))
-    pairs = np.concatenate([cross, *intra]).astype(np.uint32)
-    return gg.finalize(pairs, rng)
+    return gg.bipartite_plus(part, p_in, seed)
 
 
 @pytest.mark.parametrize("part,k", [(300, 3), (300, 4), (300, 5), (560, 3), (560, 4), (560, 5)])
@@ -251,6 +287,27 @@ def test_wide_dense_rows_vs_oracle(kcl, oracle, k):
         acc, visits = g.clique_enum(k, kcl.FRANK_WOLFE, rho=rho)
         want, wv = oracle.fw_counts(og, k, rho)
         assert visits == wv and np.array_equal(acc, want.astype(np.float64))
+
+
+@pytest.mark.parametrize("threshold", [64.0, 0.0])
+@pytest.mark.parametrize("case", WIDE)
+def test_wide_word_classes_match_reference(kcl, case, threshold):
+    """Roots in the 6-, 8-, 12- and 16-word classes at k = 6, 7, 8 against
+    the reference's own passes (FW at a rho with ties, PAVA at a level
+    permutation), with the dense counting threshold at its default and at
+    0 (every ranked root counted a wavefront per item)."""
+    gd = load_golden(case)
+    pairs = golden_pairs(gd)
+    k = int(gd["k"])
+    _say("wide", case, "threshold", threshold)
+    with kcl.Graph(pairs) as g:
+        assert g.core == int(gd["core"])
+        g.set_dense_threshold(threshold)
+        acc, visits = g.clique_enum(k, kcl.FRANK_WOLFE, rho=gd["rho"])
+        assert visits == int(gd["visits"])
+        assert np.array_equal(acc, gd["fw"].astype(np.float64))
+        acc, _ = g.clique_enum(k, kcl.PAVA_PREPROCESS, level=gd["level"])
+        assert np.array_equal(acc, gd["tent"].astype(np.float64))
 
 
 def test_counts_do_not_depend_on_dense_threshold(kcl):

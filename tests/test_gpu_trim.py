@@ -8,7 +8,7 @@ import subprocess
 import numpy as np
 import pytest
 
-from conftest import ROOT, golden_index, load_golden
+from conftest import ROOT, golden_index, golden_pairs, load_golden
 from kclistpp_amd import graphgen as gg
 
 pytestmark = pytest.mark.gpu
@@ -32,7 +32,7 @@ def test_trim_matches_reference(kcl, case):
     gd = load_golden(case)
     k = int(gd["k"])
     _say("trim", case)
-    with kcl.Trim(gd["pairs"], k) as t:
+    with kcl.Trim(golden_pairs(gd), k) as t:
         mp, nck, pk, pc, kept = t.nodes()
         assert np.array_equal(np.stack([mp, nck]).T.astype(np.uint64), gd["ckdeg"])
         assert np.array_equal(np.stack([mp[pk], pc]).T.astype(np.uint64), gd["ckcore"])

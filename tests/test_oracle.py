@@ -15,12 +15,13 @@ import math
 import numpy as np
 import pytest
 
-from conftest import golden_index, load_golden
+from conftest import golden_index, golden_pairs, load_golden
 from kclistpp_amd import graphgen as gg
 
 SIM = golden_index()["sim"]
 TRIM = golden_index()["trim"]
 BIG = golden_index()["big"]
+WIDE = golden_index()["wide"]
 
 
 def _bits(x: float) -> int:
@@ -51,7 +52,7 @@ def test_oracle_matches_reference_sim(oracle, case):
 def test_oracle_matches_reference_trim(oracle, case):
     gd = load_golden(case)
     k = int(gd["k"])
-    info, mp, nck, ick, icv, trimmed = oracle.trim(gd["pairs"], k)
+    info, mp, nck, ick, icv, trimmed = oracle.trim(golden_pairs(gd), k)
     # ckdeg: "old id, k-clique degree" per new id (kClistCoreTrim.c:419-429).
     assert np.array_equal(np.stack([mp, nck]).T.astype(np.uint64), gd["ckdeg"])
     # ckcore: peel order with running core (kClistCoreTrim.c:672-676).
@@ -79,6 +80,19 @@ def test_oracle_matches_reference_er20k(oracle, case):
     got = np.array([[r["T"], r["n"], r["m"], _bits(r["density"]), _bits(r["ub"])] for r in recs],
                    dtype=np.uint64)
     assert np.array_equal(got, gd["records"])
+
+
+@pytest.mark.parametrize("case", [c for c in WIDE if "bip340" in c])
+def test_oracle_matches_reference_wide_rows(oracle, case):
+    """Out-neighbourhoods of up to 350 members (the reference's FW pass at a
+    rho with ties, and its PAVA pass at a level permutation)."""
+    gd = load_golden(case)
+    g = oracle.dag(golden_pairs(gd))
+    k = int(gd["k"])
+    assert g.core == int(gd["core"])
+    fw, visits = oracle.fw_counts(g, k, gd["rho"])
+    assert visits == int(gd["visits"]) and np.array_equal(fw, gd["fw"])
+    assert np.array_equal(oracle.pava_counts(g, k, gd["level"]), gd["tent"])
 
 
 # ---------------------------------------------------------------------------
