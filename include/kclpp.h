@@ -126,6 +126,25 @@ kcl_status kcl_count_edges(const kcl_graph *g, uint32_t num_nodes, const uint32_
 kcl_status kcl_clique_enum(kcl_graph *g, uint32_t k, kcl_task task, const double *rho,
                            const uint32_t *level, double *acc, uint64_t *visits);
 
+/* The same pass without allocating: no counterpart in the reference.  For
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc): every intermediate
+ * (keys, counts, the task list, tickets) lives in one static device array
+ * of KCL_NOALLOC_SCRATCH_BYTES per GPU, held from library load until the
+ * process ends, and the call makes no hipMalloc/hipFree or other allocating
+ * call.  Arguments and results as kcl_clique_enum, with these limits:
+ * 24 bytes per node plus 8 per task must fit the scratch (about 2 million
+ * nodes), no root may need edge tasks (core <= the graph's
+ * kcl_graph_set_edge_split bound), and for KCL_PAVA_PREPROCESS level must
+ * be a permutation of 0..n-1 (as the reference's level always is).
+ * Sizes beyond the scratch return KCL_E_UNSUPPORTED.  Calls on one GPU
+ * share its scratch: calls that overlap on one GPU (two host threads, or
+ * another call before one returns) race and are not supported.  The
+ * scratch keeps the last call's contents; each call clears what it reads. */
+#define KCL_NOALLOC_SCRATCH_BYTES (64u * 1024u * 1024u)
+kcl_status kcl_clique_enum_noalloc(kcl_graph *g, uint32_t k, kcl_task task, const double *rho,
+                                   const uint32_t *level, double *acc, uint64_t *visits);
+
 /* ---------------------------------------------------------------------
  * Device-resident Sim-kClist++ state: rho (fp64), the per-iteration
  * counts (uint64) and the exact totals V (uint64, rho_t = 2 V_t /

@@ -128,7 +128,8 @@ EXPORTS = [
     "kcl_graph_create", "kcl_graph_destroy", "kcl_graph_get_info", "kcl_graph_rank",
     "kcl_graph_set_edge_split",
     "kcl_graph_set_dense_threshold",
-    "kcl_graph_csr", "kcl_graph_edges", "kcl_count_edges", "kcl_clique_enum", "kcl_sim_create",
+    "kcl_graph_csr", "kcl_graph_edges", "kcl_count_edges", "kcl_clique_enum",
+    "kcl_clique_enum_noalloc", "kcl_sim_create",
     "kcl_sim_destroy", "kcl_sim_step", "kcl_sim_counts", "kcl_sim_rho", "kcl_sim_set_rho",
     "kcl_sim_totals", "kcl_sim_last_pass", "kcl_sim_plan_info", "kcl_sim_root_cliques", "kcl_sim_profile_pass", "kcl_sim_find_densest", "kcl_cmpsync_run",
     "kcl_device_get_info", "kcl_version", "kcl_trim_create", "kcl_trim_destroy",
@@ -171,6 +172,7 @@ def lib() -> C.CDLL:
         "kcl_graph_edges": ([vp, vp], C.c_int),
         "kcl_count_edges": ([vp, u32, vp, C.POINTER(u32)], C.c_int),
         "kcl_clique_enum": ([vp, u32, C.c_int, vp, vp, vp, C.POINTER(u64)], C.c_int),
+        "kcl_clique_enum_noalloc": ([vp, u32, C.c_int, vp, vp, vp, C.POINTER(u64)], C.c_int),
         "kcl_sim_create": ([vp, u32, C.POINTER(_SimOpts), C.POINTER(vp)], C.c_int),
         "kcl_sim_destroy": ([vp], None),
         "kcl_sim_step": ([vp, u32], C.c_int),
@@ -365,6 +367,20 @@ class Graph:
         v = C.c_uint64(0)
         _check(lib().kcl_clique_enum(self.handle, k, task, _ptr(rho_a), _ptr(lev_a), _ptr(acc),
                                      C.byref(v)))
+        return acc[:self.n], int(v.value)
+
+    def clique_enum_noalloc(self, k: int, task: int, rho: np.ndarray | None = None,
+                            level: np.ndarray | None = None, acc: np.ndarray | None = None):
+        """kcl_clique_enum_noalloc: the same pass with no device allocation
+        (intermediates in the library's static scratch); returns (acc,
+        visits)."""
+        if acc is None:
+            acc = np.zeros(max(self.n, 1), np.float64)
+        rho_a = None if rho is None else np.ascontiguousarray(rho, np.float64)
+        lev_a = None if level is None else np.ascontiguousarray(level, np.uint32)
+        v = C.c_uint64(0)
+        _check(lib().kcl_clique_enum_noalloc(self.handle, k, task, _ptr(rho_a), _ptr(lev_a),
+                                             _ptr(acc), C.byref(v)))
         return acc[:self.n], int(v.value)
 
 

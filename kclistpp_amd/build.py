@@ -36,7 +36,7 @@ HIP_FLAGS = [
 ]
 
 SOURCES = ["host_io.cpp", "host_order.cpp", "graph.hip", "pass.hip", "plan.hip", "sim.hip", "batch.hip",
-           "costmodel.hip", "trim.cpp", "exact.cpp"]
+           "costmodel.hip", "trim.cpp", "exact.cpp", "noalloc.hip"]
 CLIS = {
     "kCDensestCmpSync": "cli_cmpsync.c",
     "kClistCoreTrim": "cli_trim.c",
