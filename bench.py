@@ -1,14 +1,16 @@
 """Benchmark: k-clique visits/s of one Sim-kClist++ iteration on one MI355X.
 
-Workload (the reference's k=5 case): seeded Chung-Lu graph, n = 1,000,000,
-m = 16,000,000, degree exponent 2.3, largest expected degree 20,000, plus a
-planted 150-vertex community with internal edge probability 0.8; k = 5.
+Default workload (the k = 6 case of the exact pipeline): seeded Chung-Lu
+graph, n = 4,000,000, m = 60,000,000, degree exponent 2.1, largest expected
+degree 40,000, plus a planted 200-vertex near-clique with internal edge
+probability 0.9; k = 6, roots ordered and split by the learned cost model.
 A step is one iteration t of kCDensestCmpSync.c:396-404: the enumeration
 pass over all k-cliques with the per-clique min-rho credit, then the fp64
 rho update.  The graph, rho and the counts are resident in HBM when the
-timed region starts.
+timed region starts.  The same run times the plain (d+^2) schedule on the
+same states, and a short leg of the k = 5 workload (Chung-Lu 1M/16M).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload cl1m]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload cl4m]
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
@@ -63,59 +65,120 @@ def algorithmic_bytes(cd: np.ndarray, adj: np.ndarray, roots: np.ndarray) -> int
     return 16 * int(roots.sum()) + int(per_edge[roots[src]].sum()) + 8 * n
 
 
-def profiled(workload: str, key: str):
+def profiled(workload: str, key: str, text: bool = False):
     """A figure from the committed rocprofv3 summary of this workload
-    (traffic_bytes_per_pass: FETCH_SIZE + WRITE_SIZE scaled by the
-    copy-kernel factors; lds_frac_upper_per_pass), or None."""
+    (profiles/pass_kernel_<workload>.md, lines "key: value ..."), or None:
+    traffic_bytes_per_pass (FETCH_SIZE + WRITE_SIZE scaled by the copy
+    kernel's factors), valu_busy_frac, lds_busy_frac, lds_bank_conflict_frac,
+    hbm_traffic_frac, bound."""
     path = os.path.join(ROOT, "profiles", f"pass_kernel_{workload}.md")
     if not os.path.exists(path):
         return None
     for line in open(path):
         if line.startswith(key + ":"):
-            return float(line.split(":")[1].split()[0])
+            v = line.split(":", 1)[1].split()[0]
+            return v if text else float(v)
     return None
 
 
-def cpu_baseline(pairs: np.ndarray, k: int, target_s: float = 12.0):
-    """Times the reference's own CDF_CliqueEnum(FRANK_WOLFE) (built from its
-    sources into oracle/_ref) on a random sample of root edges at rho = 0,
-    with all host threads.  Falls back to the oracle's single-threaded
-    restatement when the reference build is absent."""
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+def cpu_model() -> str:
     try:
-        from oracle import RefHarness
-        ref = RefHarness()
-    except Exception as e:  # pragma: no cover - reference build missing
-        log("reference build unavailable:", e)
-        return cpu_baseline_port(pairs, k, target_s)
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def host_threads() -> int:
+    return int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+
+
+def cpu_baseline(cd: np.ndarray, adj: np.ndarray, root_cliques: np.ndarray, k: int,
+                 budget_s: float = 30.0):
+    """Whole-pass time of the reference's own CDF_CliqueEnum(FRANK_WOLFE)
+    (built from its sources into oracle/_ref) on all host threads, estimated
+    from a bounded stratified sample of its root edges at rho = 0.
+
+    The reference runs one task per DAG edge (s, t) under a dynamic OpenMP
+    schedule (kCDensestCmpSync.c:300-305).  Edges are grouped by their
+    source's k-clique count (exact, from the GPU plan): stratum 0 holds the
+    edges of roots without a k-clique, stratum b >= 1 those of roots with
+    [2^(b-1), 2^b) cliques.  Each stratum is sampled uniformly, in batches
+    sized for about half a second of all threads and doubling within the
+    stratum's share of the budget (a small stratum is run whole).  The
+    sample's cost is taken in thread-seconds: a large batch runs on all
+    threads (wall x threads), a small one edge by edge, one edge per thread
+    under its own clock.  A stratum's time is its thread-seconds scaled by
+    its edge count and divided by the thread count (ideal scaling, which
+    favours the CPU); value = the pass's k-cliques / the summed estimate."""
+    threads = host_threads()
+    from oracle import RefHarness
+    ref = RefHarness()
     ref.threads(threads)
     t0 = time.time()
-    h = ref.load_pairs(pairs)  # the reference's SortByCore/Relabel/MakeGraph
-    log(f"cpu baseline: reference graph built in {time.time() - t0:.1f}s")
-    info = ref.info(h)
+    h = ref.load_csr(cd, adj)  # the same DAG as the reference's MakeGraph (tests check it)
     ref.start(h, k)
-    e = info["e"]
+    log(f"cpu baseline: reference graph from the CSR in {time.time() - t0:.1f}s")
+    n = cd.size - 1
+    dout = np.diff(cd).astype(np.int64)
+    rc = root_cliques[:n].astype(np.float64)
+    stratum = np.zeros(n, np.int64)
+    nz = rc > 0
+    stratum[nz] = 1 + np.floor(np.log2(rc[nz])).astype(np.int64)
+    levels = [b for b in np.unique(stratum) if dout[stratum == b].sum() > 0]
+    share = budget_s / max(len(levels), 1)
     rng = np.random.default_rng(12345)
-    chunk = 256
-    spent, visits, tried = 0.0, 0.0, 0
-    perm = rng.permutation(e).astype(np.uint32)
-    # Grow the sample (chunks doubling from 256 root edges) until about
-    # target_s seconds of reference work.
-    while spent < target_s and tried < e:
-        take = int(min(e - tried, chunk))
-        roots = np.sort(perm[tried:tried + take])
-        s, v = ref.time_fw_roots(h, roots)
-        spent += s
-        visits += v
-        tried += take
-        log(f"cpu baseline: {tried} roots, {visits:.3e} visits, {spent:.1f}s")
-        chunk *= 2
+    total_s, sampled_all, wall_all = 0.0, 0, 0.0
+    per_thread_rate = 4e7  # visits/s, only to size the first batch
+    big_batch = 64 * threads  # from here on, batches run on all threads at once
+    for b in levels:
+        roots = np.nonzero(stratum == b)[0]
+        nb = int(dout[roots].sum())
+        vb = float(rc[roots].sum())
+        csum = np.cumsum(dout[roots])
+        batch = int(min(max(per_thread_rate * threads * 0.5 / max(vb / nb, 1.0), threads),
+                        1 << 22))
+        thread_s, wall, taken = 0.0, 0.0, 0
+        while wall < share and taken < nb:
+            if nb <= batch and taken == 0:  # the whole stratum
+                idx = np.nonzero(np.repeat(stratum == b, dout))[0] if roots.size > 50000 else \
+                    np.concatenate([np.arange(cd[r], cd[r + 1]) for r in roots])
+            else:
+                pos = rng.integers(0, nb, min(batch, nb))
+                ri = np.searchsorted(csum, pos, side="right")
+                idx = cd[roots[ri]] + (pos - (csum[ri] - dout[roots[ri]]))
+            idx = np.sort(idx).astype(np.uint32)
+            if idx.size >= big_batch:
+                sec, _ = ref.time_fw_roots(h, idx)
+                thread_s += sec * threads
+            else:
+                sec, per, _ = ref.time_fw_each(h, idx)
+                thread_s += float(per.sum())
+            wall += sec
+            taken += idx.size
+            batch *= 2
+            if idx.size == nb and taken == nb:
+                break
+        est = thread_s * nb / taken / threads if taken else 0.0
+        total_s += est
+        sampled_all += taken
+        wall_all += wall
+        log(f"cpu baseline: stratum {b}: {roots.size} roots, {nb} edges, {vb:.3e} cliques; "
+            f"sampled {taken} edges, {thread_s:.2f} thread-s in {wall:.2f}s -> {est:.3e}s")
     ref.free(h)
-    return {"value": visits / spent if spent > 0 else 0.0, "unit": "k-clique visits/s",
-            "cores": threads, "kind": "reference",
-            "sample": f"CDF_CliqueEnum(FRANK_WOLFE) of the reference over {tried} of {e} root "
-                      f"edges drawn uniformly (seed 12345), rho = 0, {threads} OpenMP threads, "
-                      f"{spent:.1f}s"}
+    total_v = float(rc.sum())
+    model = cpu_model()
+    return {"value": total_v / total_s if total_s > 0 else 0.0, "unit": "k-clique visits/s",
+            "cores": threads, "kind": "reference", "cpu_model": model,
+            "pass_seconds_estimate": total_s,
+            "sample": f"CDF_CliqueEnum(FRANK_WOLFE) of the reference (built from its sources) "
+                      f"at rho = 0, {threads} OpenMP threads of a {model}: {sampled_all} of "
+                      f"{int(dout.sum())} root edges in {len(levels)} strata by their root's "
+                      f"k-clique count (log2), {wall_all:.1f}s of sampling; per stratum, "
+                      f"sampled thread-seconds x edges / sampled edges / threads; whole-pass "
+                      f"estimate {total_s:.4g}s for {total_v:.4g} cliques"}
 
 
 def cpu_baseline_port(pairs, k, target_s):  # pragma: no cover - fallback only
@@ -316,10 +379,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="cl1m", choices=sorted(WORKLOADS) + ["batch512"])
+    ap.add_argument("--workload", default="cl4m", choices=sorted(WORKLOADS) + ["batch512"])
     ap.add_argument("--no-tune", action="store_true",
                     help="batch512: keep the default row width and occupancy")
-    ap.add_argument("--schedule", default="counted", choices=["counted", "learned", "degree"],
+    ap.add_argument("--schedule", default="learned", choices=["counted", "learned", "degree"],
                     help="root order/splitting: counting-pass costs, the learned cost "
                          "model, or plain d+^2 (counts are identical under all three)")
     ap.add_argument("--model", default=None,
@@ -329,7 +392,21 @@ def main():
                     help="exact mode: seconds before giving up on a certificate")
     ap.add_argument("--mode", default="sim", choices=["sim", "exact"],
                     help="sim: visits/s of one iteration (default); exact: time-to-exact")
+    ap.add_argument("--plain-steps", type=int, default=2,
+                    help="iterations of the plain (d+^2) schedule timed on the main run's "
+                         "states (0: skip)")
+    ap.add_argument("--secondary", default="cl1m",
+                    help="a second workload run for a few steps ('' to skip)")
+    ap.add_argument("--cpu-budget", type=float, default=24.0,
+                    help="seconds of reference work in the CPU baseline's sample")
+    ap.add_argument("--cache", default="",
+                    help="directory for the generated edges (reused by later runs)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB on the device (counter calibration)")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               f"(the multi-GPU sharding is described, not built)", file=sys.stderr)
@@ -344,13 +421,118 @@ def main():
     if args.workload == "batch512":
         run_batch(args, kcl, gg)
         return
+    run_sim(args, kcl, gg)
+
+
+def timed_steps(sim, t: int, steps: int, cliques: int, label: str):
+    """steps synchronous iterations from t+1; returns (t, seconds, visits,
+    enumeration-kernel ms summed, pass ms list)."""
+    visits, kern_ms, passes = 0, 0.0, []
+    t_start = time.perf_counter()
+    for _ in range(steps):
+        t += 1
+        sim.step(t)  # synchronous: returns after the pass and update completed
+        v, kms, pms = sim.last_pass()
+        if v != cliques:
+            raise RuntimeError(f"{label}: step {t} visited {v} cliques, the graph has {cliques}")
+        visits += v
+        kern_ms += kms
+        passes.append(pms)
+    return t, time.perf_counter() - t_start, visits, kern_ms, passes
+
+
+def plain_leg(kcl, g, k, sim, t: int, steps: int):
+    """The plain schedule (d+^2 order and splits) on the same states as the
+    main schedule: per step, the plain sim takes the main sim's rho, both run
+    iteration t, their counts must agree, and their pass times are kept."""
+    t0 = time.time()
+    plain = kcl.Sim(g, k, schedule=kcl.SCHED_DEGREE)
+    plan_s = time.time() - t0
+    main_ms, plain_ms = [], []
+    for _ in range(steps):
+        t += 1
+        plain.set_rho(sim.rho())
+        sim.step(t)
+        plain.step(t)
+        if not np.array_equal(sim.counts(), plain.counts()):
+            raise RuntimeError(f"plain schedule counts differ at t={t}")
+        main_ms.append(sim.last_pass()[2])
+        plain_ms.append(plain.last_pass()[2])
+        log(f"bench: t={t} main pass {main_ms[-1]:.2f}ms, plain pass {plain_ms[-1]:.2f}ms")
+    plain.close()
+    m, p = float(np.mean(main_ms)), float(np.mean(plain_ms))
+    return t, {"steps": steps, "t_from": t - steps + 1, "plan_s": round(plan_s, 2),
+               "main_pass_ms": round(m, 3), "plain_pass_ms": round(p, 3),
+               "main_vs_plain": round(p / m, 4) if m > 0 else None}
+
+
+def workload_pairs(args, gg, wname: str, seed: int, sd: int) -> np.ndarray:
+    """The workload's edges: generated, or loaded from --cache DIR where an
+    earlier run stored them (the generator is seeded, so both are the same
+    array)."""
+    path = os.path.join(args.cache, f"{wname}_s{seed}_d{sd}.npy") if args.cache else None
+    if path and os.path.exists(path):
+        return np.load(path, allow_pickle=False)
+    pairs = gg.workload(wname, seed=seed, scale_down=sd)
+    if path:
+        os.makedirs(args.cache, exist_ok=True)
+        np.save(path + ".tmp.npy", pairs)
+        os.replace(path + ".tmp.npy", path)
+    return pairs
+
+
+def secondary_leg(args, kcl, gg, name: str, steps: int = 10, warmup: int = 2):
+    """A short run of another workload under the same schedule."""
+    wname, seed, sd, k, desc = WORKLOADS[name]
+    pairs = workload_pairs(args, gg, wname, seed, sd)
+    model = load_model(args, kcl)
+    with kcl.Graph(pairs) as g:
+        del pairs
+        sim = kcl.Sim(g, k, schedule=kcl.SCHEDULES[args.schedule], model=model)
+        plan = sim.plan_info()
+        t, _, _, _, _ = timed_steps(sim, 0, warmup, plan["cliques"], name)
+        t, sec, visits, kern_ms, _ = timed_steps(sim, t, steps, plan["cliques"], name)
+        sim.close()
+    if model is not None:
+        model.close()
+    out = {"workload": f"{name}: {desc}", "schedule": args.schedule, "steps": steps,
+           "value": visits / sec if sec > 0 else 0.0, "unit": "k-clique visits/s",
+           "ms_per_step": 1000.0 * sec / steps, "kernel_ms": kern_ms / steps,
+           "visits_per_step": plan["cliques"]}
+    log(f"bench: secondary {name}: {out}")
+    return out
+
+
+def roofline_of(workload: str, alg_bytes: int, kern_avg_ms: float) -> dict:
+    """The dominant kernel's roofline: algorithmic HBM bytes per pass over
+    the enumeration kernels' time per pass (HIP events on the pass's
+    stream), plus what the committed counter profile of this workload
+    measured (HBM traffic, VALU and LDS busy fractions, the bound)."""
+    achieved = alg_bytes / (kern_avg_ms * 1e-3) / 1e9 if kern_avg_ms > 0 else 0.0
+    traffic = profiled(workload, "traffic_bytes_per_pass")
+    out = {"bound": profiled(workload, "bound", text=True) or "hbm",
+           "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS,
+           "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6),
+           "traffic": int(traffic) if traffic is not None else None,
+           "kernel": "pass_kernel (all word classes of one enumeration pass)",
+           "kernel_ms": round(kern_avg_ms, 4), "algorithmic_bytes": alg_bytes}
+    for key in ("valu_busy_frac", "lds_busy_frac", "lds_bank_conflict_frac",
+                "hbm_traffic_frac"):
+        v = profiled(workload, key)
+        if v is not None:
+            out[key] = v
+    return out
+
+
+def run_sim(args, kcl, gg):
     wname, seed, sd, k, desc = WORKLOADS[args.workload]
     log(f"bench: generating {args.workload} ({desc})")
     t0 = time.time()
-    pairs = gg.workload(wname, seed=seed, scale_down=sd)
+    pairs = workload_pairs(args, gg, wname, seed, sd)
     log(f"bench: {pairs.shape[0]} edges in {time.time() - t0:.1f}s; building the graph")
     t0 = time.time()
     g = kcl.Graph(pairs)
+    del pairs
     log(f"bench: graph n={g.n} m={g.m} core={g.core} built in {time.time() - t0:.1f}s")
     cd, adj = g.csr()
     sched = kcl.SCHEDULES[args.schedule]
@@ -358,45 +540,35 @@ def main():
     t0 = time.time()
     sim = kcl.Sim(g, k, schedule=sched, model=model)
     plan = sim.plan_info()
-    alg_bytes = algorithmic_bytes(cd, adj, sim.root_cliques() > 0)
+    root_cliques = sim.root_cliques()
+    alg_bytes = algorithmic_bytes(cd, adj, root_cliques > 0)
     log(f"bench: plan ready in {time.time() - t0:.1f}s: {plan}; warmup {args.warmup}")
     t = 0
     for _ in range(args.warmup):
-        t += 1
-        sim.step(t)
-        v, kms, pms = sim.last_pass()
-        log(f"bench: warmup t={t} visits={v} pass={pms:.2f}ms kernels={kms:.2f}ms")
-    visits, kern_ms = 0, 0.0
-    t_start = time.perf_counter()
-    for _ in range(args.steps):
-        t += 1
-        sim.step(t)  # synchronous: returns after the pass and update completed
-        v, kms, pms = sim.last_pass()
-        if v != plan["cliques"]:
-            raise RuntimeError(f"step {t} visited {v} cliques, the graph has {plan['cliques']}")
-        visits += v
-        kern_ms += kms
-    elapsed = time.perf_counter() - t_start
-    log(f"bench: {args.steps} steps in {elapsed:.3f}s, {visits} visits")
+        t, _, _, _, ps = timed_steps(sim, t, 1, plan["cliques"], "warmup")
+        log(f"bench: warmup t={t} pass={ps[0]:.2f}ms")
+    t, elapsed, visits, kern_ms, passes = timed_steps(sim, t, args.steps, plan["cliques"], "bench")
+    log(f"bench: {args.steps} steps in {elapsed:.3f}s, {visits} visits; passes "
+        f"{min(passes):.2f}-{max(passes):.2f}ms")
     ms_per_step = 1000.0 * elapsed / max(args.steps, 1)
     value = visits / elapsed if elapsed > 0 else 0.0
     kern_avg_ms = kern_ms / max(args.steps, 1)
-    achieved = alg_bytes / (kern_avg_ms * 1e-3) / 1e9 if kern_avg_ms > 0 else 0.0
-    roofline = {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6),
-                "traffic": (lambda v: int(v) if v is not None else None)(
-                    profiled(args.workload, "traffic_bytes_per_pass")),
-                "lds_frac_upper": profiled(args.workload, "lds_frac_upper_per_pass"),
-                "kernel": "pass_kernel (all word classes of one enumeration pass)",
-                "kernel_ms": round(kern_avg_ms, 4), "algorithmic_bytes": alg_bytes}
-    cpu = None
-    if not args.no_cpu_baseline:
-        log("bench: timing the reference on the host cores")
-        cpu = cpu_baseline(pairs, k)
+    roofline = roofline_of(args.workload, alg_bytes, kern_avg_ms)
+    plain = None
+    if args.plain_steps > 0 and args.schedule != "degree":
+        t, plain = plain_leg(kcl, g, k, sim, t, args.plain_steps)
     sim.close()
     if model is not None:
         model.close()
     g.close()
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("bench: timing the reference on the host cores")
+        cpu = cpu_baseline(cd, adj, root_cliques, k, budget_s=args.cpu_budget)
+    del cd, adj
+    secondary = None
+    if args.secondary and args.secondary != args.workload:
+        secondary = secondary_leg(args, kcl, gg, args.secondary)
     out = {
         "metric": "k-clique visits/sec per kClist++ iteration",
         "value": value,
@@ -411,13 +583,31 @@ def main():
         "dtype": "u64",
         "data": "synthetic (seeded generator, kclistpp_amd/graphgen.py)",
         "config": {"workload": f"{args.workload}: {desc}", "k": k, "n": g.n, "m": g.m,
-                   "core": g.core, "roots": plan["roots_kept"], "visits_per_step": visits // max(args.steps, 1),
+                   "core": g.core, "roots": plan["roots_kept"],
+                   "visits_per_step": visits // max(args.steps, 1),
                    "schedule": args.schedule, "parallelism": "1 GPU",
+                   "t_timed": [args.warmup + 1, args.warmup + args.steps],
                    "plan": plan},
+        "plain_schedule": plain,
+        "secondary": secondary,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def copy_probe(mib: int) -> None:
+    """A plain device copy of `mib` MiB (torch), for scaling the FETCH_SIZE
+    and WRITE_SIZE counters: rocprofv3 --pmc over this run reads what the
+    counters report for a known number of bytes."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    log(f"copy probe: {mib} MiB copied")
 
 
 if __name__ == "__main__":

@@ -209,6 +209,8 @@ class RefHarness:
         L.kclref_load.restype = C.c_void_p
         L.kclref_load_pairs.argtypes = [_u32p, C.c_uint]
         L.kclref_load_pairs.restype = C.c_void_p
+        L.kclref_load_csr.argtypes = [C.c_uint, _u32p, _u32p]
+        L.kclref_load_csr.restype = C.c_void_p
         for f in ("kclref_n_in", "kclref_n", "kclref_e", "kclref_core"):
             getattr(L, f).argtypes = [C.c_void_p]
             getattr(L, f).restype = C.c_uint
@@ -224,6 +226,9 @@ class RefHarness:
         L.kclref_pava_level.argtypes = [C.c_void_p, _u32p, _f64p]
         L.kclref_time_fw_roots.argtypes = [C.c_void_p, _u32p, C.c_uint, C.POINTER(C.c_double)]
         L.kclref_time_fw_roots.restype = C.c_double
+        L.kclref_time_fw_each.argtypes = [C.c_void_p, _u32p, C.c_uint, _f64p,
+                                          C.POINTER(C.c_double)]
+        L.kclref_time_fw_each.restype = C.c_double
         L.kclref_free.argtypes = [C.c_void_p]
         self.L = L
 
@@ -238,6 +243,15 @@ class RefHarness:
         its reader would build from their text)."""
         p = np.ascontiguousarray(pairs, np.uint32)
         h = self.L.kclref_load_pairs(p, len(p))
+        if not h:
+            raise RuntimeError("reference load failed")
+        return h
+
+    def load_csr(self, cd: np.ndarray, adj: np.ndarray):
+        """The reference's Graph from a DAG CSR (no degeneracy ordering)."""
+        cd = np.ascontiguousarray(cd, np.uint32)
+        adj = np.ascontiguousarray(adj if adj.size else np.zeros(1, np.uint32), np.uint32)
+        h = self.L.kclref_load_csr(cd.size - 1, cd, adj)
         if not h:
             raise RuntimeError("reference load failed")
         return h
@@ -304,6 +318,15 @@ class RefHarness:
         roots = np.ascontiguousarray(roots, np.uint32)
         s = self.L.kclref_time_fw_roots(h, roots, roots.size, C.byref(v))
         return float(s), float(v.value)
+
+    def time_fw_each(self, h, roots: np.ndarray):
+        """Per-edge thread-seconds of CDF_CliqueEnum(FRANK_WOLFE), edges run
+        one per thread: (wall seconds, per-edge seconds, visits)."""
+        roots = np.ascontiguousarray(roots, np.uint32)
+        sec = np.zeros(max(roots.size, 1), np.float64)
+        v = C.c_double(0)
+        wall = self.L.kclref_time_fw_each(h, roots, roots.size, sec, C.byref(v))
+        return float(wall), sec[:roots.size], float(v.value)
 
     def free(self, h):
         self.L.kclref_free(h)

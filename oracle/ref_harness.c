@@ -83,6 +83,35 @@ void *kclref_load_pairs(const unsigned *pairs, unsigned m) {
   return h;
 }
 
+/* The reference's Graph filled from a DAG CSR the caller holds (rows of
+ * smaller ids, ascending: the layout MakeGraph builds, Graph.c:120-156),
+ * with edges in (s, t) order.  The benchmark's CPU baseline uses it on a
+ * graph whose CSR the tests show equal to the reference's, to skip a second
+ * degeneracy ordering of a large input.  No ranks are kept. */
+void *kclref_load_csr(unsigned n, const unsigned *cd, const unsigned *adj) {
+  Graph *g = (Graph *)malloc(sizeof(Graph));
+  g->n = n;
+  g->e = cd[n];
+  g->cd = (unsigned *)malloc((n + 1) * sizeof(unsigned));
+  memcpy(g->cd, cd, (n + 1) * sizeof(unsigned));
+  g->adj = (unsigned *)malloc((g->e ? g->e : 1) * sizeof(unsigned));
+  memcpy(g->adj, adj, g->e * sizeof(unsigned));
+  g->edges = (Edge *)malloc((g->e ? g->e : 1) * sizeof(Edge));
+  g->core = 0;
+  for (unsigned s = 0; s < n; ++s) {
+    unsigned d = cd[s + 1] - cd[s];
+    if (d > g->core) g->core = d;
+    for (unsigned j = cd[s]; j < cd[s + 1]; ++j) {
+      g->edges[j].s = s;
+      g->edges[j].t = adj[j];
+    }
+  }
+  kclref *h = (kclref *)calloc(1, sizeof(kclref));
+  h->n_in = n;
+  h->g = g;
+  return h;
+}
+
 unsigned kclref_n_in(void *p) { return ((kclref *)p)->n_in; }
 unsigned kclref_n(void *p) { return ((kclref *)p)->g->n; }
 unsigned kclref_e(void *p) { return ((kclref *)p)->g->e; }
@@ -184,6 +213,40 @@ double kclref_time_fw_roots(void *p, const unsigned *roots, unsigned nroots, dou
   g->edges = all;
   g->e = e_all;
   free(sub);
+  double s = 0;
+  for (unsigned i = 0; i < g->n; ++i) s += rho_add[i];
+  *visits = s;
+  return (double)(b.tv_sec - a.tv_sec) + 1e-9 * (double)(b.tv_nsec - a.tv_nsec);
+}
+
+/* Thread-seconds of each root edge alone: the sample's edges are handed to
+ * the host threads one at a time (dynamic schedule), and each thread runs
+ * the reference's CDF_CliqueEnum(FRANK_WOLFE) over a one-edge view of the
+ * graph (its own parallel region then has one thread: nesting is off) under
+ * its own clock.  sec_out[i] receives edge i's seconds; returns the wall
+ * seconds of the whole sample, *visits the cliques credited. */
+double kclref_time_fw_each(void *p, const unsigned *roots, unsigned nroots, double *sec_out,
+                           double *visits) {
+  kclref *h = (kclref *)p;
+  Graph *g = h->g;
+  for (unsigned i = 0; i < g->n; ++i) rho_add[i] = 0;
+  int levels = omp_get_max_active_levels();
+  omp_set_max_active_levels(1);
+  struct timespec a, b;
+  clock_gettime(CLOCK_MONOTONIC, &a);
+  #pragma omp parallel for schedule(dynamic, 1)
+  for (unsigned i = 0; i < nroots; ++i) {
+    Graph one = *g;
+    one.edges = g->edges + roots[i];
+    one.e = 1;
+    struct timespec x, y;
+    clock_gettime(CLOCK_MONOTONIC, &x);
+    CDF_CliqueEnum(&one, (unsigned char)h->k, REF_FRANK_WOLFE);
+    clock_gettime(CLOCK_MONOTONIC, &y);
+    sec_out[i] = (double)(y.tv_sec - x.tv_sec) + 1e-9 * (double)(y.tv_nsec - x.tv_nsec);
+  }
+  clock_gettime(CLOCK_MONOTONIC, &b);
+  omp_set_max_active_levels(levels);
   double s = 0;
   for (unsigned i = 0; i < g->n; ++i) s += rho_add[i];
   *visits = s;

@@ -270,3 +270,29 @@ def test_reference_pairs_loader_matches_its_reader(tmp_path):
     assert ref.info(a) == ref.info(b)
     assert np.array_equal(ref.rank(a), ref.rank(b))
     assert np.array_equal(ref.edges(a), ref.edges(b))
+
+
+def test_reference_csr_loader_matches_its_own_build():
+    """The reference Graph filled from a CSR (the benchmark's CPU baseline
+    path) enumerates like the one its own SortByCore/MakeGraph builds."""
+    from oracle import RefHarness
+    pairs, rng = gg.chung_lu(2000, 12000, 2.3, 200.0, 7)
+    pairs = gg.finalize(gg.plant_community(pairs, 2000, 30, 0.8, rng), rng)
+    ref = RefHarness()
+    a = ref.load_pairs(pairs)
+    cd, adj = ref.csr(a)
+    b = ref.load_csr(cd, adj)
+    assert ref.info(a)["core"] == ref.info(b)["core"] and ref.info(b)["e"] == adj.size
+    assert np.array_equal(ref.edges(a), ref.edges(b))
+    ref.start(a, 4)  # the reference's rho/rho_add are globals: one graph at a time
+    ca, _ = ref.step(a, 1)
+    ref.start(b, 4)
+    cb, _ = ref.step(b, 1)
+    # Per-edge timing (the benchmark's CPU baseline) visits what a batch does.
+    idx = np.arange(0, adj.size, 7, dtype=np.uint32)
+    _, va = ref.time_fw_roots(b, idx)
+    wall, per, vb = ref.time_fw_each(b, idx)
+    assert va == vb and per.size == idx.size and np.all(per >= 0) and wall > 0
+    assert np.array_equal(ca, cb) and ca.sum() > 0
+    ref.free(a)
+    ref.free(b)
