@@ -549,7 +549,7 @@ def run_sim(args, kcl, gg):
         log(f"bench: warmup t={t} pass={ps[0]:.2f}ms")
     t, elapsed, visits, kern_ms, passes = timed_steps(sim, t, args.steps, plan["cliques"], "bench")
     log(f"bench: {args.steps} steps in {elapsed:.3f}s, {visits} visits; passes "
-        f"{min(passes):.2f}-{max(passes):.2f}ms")
+        f"{min(passes or [0]):.2f}-{max(passes or [0]):.2f}ms")
     ms_per_step = 1000.0 * elapsed / max(args.steps, 1)
     value = visits / elapsed if elapsed > 0 else 0.0
     kern_avg_ms = kern_ms / max(args.steps, 1)

@@ -42,7 +42,11 @@
 // nwin.  A root whose expected common out-set per item is large is counted
 // a wavefront per item: the sets of the outer levels are the same in
 // every lane (scalar registers), and in the last two levels lane l takes
-// member 64j + l of each word j and counts its row over words >= j.  Rows
+// member 64j + l of each word j and counts its row over words >= j.  The
+// level above those walks its members word by word with the word index a
+// compile-time constant: a member's row is zero below its own word, so the
+// next set is formed from that word on (row words moved to scalar
+// registers, one scalar AND each) and its edges counted from there.  Rows
 // of 8 or more words (roots inside a dense core) are always counted so.
 #pragma once
 
@@ -182,25 +186,82 @@ __device__ __forceinline__ void to_sgpr(unsigned long long (&S)[W]) {
   }
 }
 
-// This lane's share of the edges inside S: for each non-zero word j of S,
-// lane l counts the row of member 64j + l inside S (words j and above
-// only) and keeps the count if that member is in S.  Every lane loads (rows
-// past the root's members hold stale bits and are masked out the same
-// way), so the loads of successive words need no branch between them.
-template <int W>
-__device__ __forceinline__ uint32_t wave_edges(const unsigned long long (&S)[W],
-                                               const unsigned long long *rows, uint32_t lane) {
+// v if this lane's bit of mask is set, else 0: the set word itself is the
+// lane mask of v_cndmask (lane l holds member 64 j + l of word j).
+// (One v_cndmask with the scalar mask operand; the shift-and-compare form
+// takes four instructions.  mask must be wave-uniform: a scalar register
+// pair.)
+__device__ __forceinline__ uint32_t keep_if_member(uint32_t v, unsigned long long mask) {
+  uint32_t r;
+  asm("v_cndmask_b32_e64 %0, 0, %1, %2" : "=v"(r) : "v"(v), "s"(mask));
+  return r;
+}
+
+// A row word that is the same in every lane, moved to scalar registers.
+__device__ __forceinline__ unsigned long long uniform_word(unsigned long long x) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)x);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(x >> 32));
+  return ((unsigned long long)hi << 32) | lo;
+}
+
+// This lane's share of the edges inside S, from word J0 on (the words of S
+// below J0 are zero): for each non-zero word j of S, lane l counts the row
+// of member 64j + l inside S (words j and above only) and keeps the count
+// if that member is in S.  Every lane loads (rows past the root's members
+// hold stale bits and are masked out the same way), so the loads of
+// successive words need no branch between them.
+template <int W, int J0>
+__device__ __forceinline__ uint32_t wave_edges_from(const unsigned long long (&S)[W],
+                                                    const unsigned long long *rows,
+                                                    uint32_t lane) {
   uint32_t c = 0;
 #pragma unroll
-  for (int j = 0; j < W; ++j) {
+  for (int j = J0; j < W; ++j) {
     if (S[j] == 0) continue;
     const unsigned long long *r = rows + (uint32_t)(64 * j + lane) * row_stride<W>();
     uint32_t cj = 0;
 #pragma unroll
     for (int jj = j; jj < W; ++jj) cj += (uint32_t)__popcll(S[jj] & r[jj]);
-    c += ((S[j] >> lane) & 1ull) ? cj : 0u;
+    c += keep_if_member(cj, S[j]);
   }
   return c;
+}
+
+// This lane's share of the triangles inside S, for the members w of S in
+// words WI and above: the set of w's later neighbours, S & A'[w], has no
+// member below w's own word, so it is formed (and moved to scalar
+// registers) from word WI on, and its edges are counted from there.
+template <int W, int WI>
+__device__ __forceinline__ uint32_t wave_triangles_from(const unsigned long long (&S)[W],
+                                                        const unsigned long long *rows,
+                                                        uint32_t lane) {
+  if constexpr (WI >= W) {
+    return 0;
+  } else {
+    uint32_t total = 0;
+    unsigned long long bits = S[WI];
+    while (bits) {
+      const uint32_t w = (uint32_t)WI * 64 + (uint32_t)__builtin_ctzll(bits);
+      bits &= bits - 1;
+      const unsigned long long *r = rows + w * row_stride<W>();
+      unsigned long long Q[W];
+      uint32_t size = 0;
+#pragma unroll
+      for (int j = 0; j < W; ++j) {
+        if (j < WI) {
+          Q[j] = 0;
+        } else {
+          // The row word moves to scalar registers first, so the
+          // intersection is one scalar AND.
+          Q[j] = S[j] & uniform_word(r[j]);
+          size += (uint32_t)__popcll(Q[j]);
+        }
+      }
+      if (size < 2) continue;
+      total += wave_edges_from<W, WI>(Q, rows, lane);
+    }
+    return total + wave_triangles_from<W, WI + 1>(S, rows, lane);
+  }
 }
 
 // This lane's share of the J-cliques inside S (J >= 2): the members of S
@@ -210,7 +271,9 @@ template <int W, int J>
 __device__ __forceinline__ uint64_t wave_cliques(const unsigned long long (&S)[W],
                                                  const unsigned long long *rows, uint32_t lane) {
   if constexpr (J == 2) {
-    return wave_edges<W>(S, rows, lane);
+    return wave_edges_from<W, 0>(S, rows, lane);
+  } else if constexpr (J == 3) {
+    return wave_triangles_from<W, 0>(S, rows, lane);
   } else {
     uint64_t total = 0;
 #pragma unroll 1
@@ -222,10 +285,10 @@ __device__ __forceinline__ uint64_t wave_cliques(const unsigned long long (&S)[W
         const uint32_t w = (uint32_t)wi * 64 + (uint32_t)__builtin_ctzll(bits);
         bits &= bits - 1;
         const unsigned long long *r = rows + w * row_stride<W>();
+        // Words below w's own are zero in its row (ranked members).
         unsigned long long Q[W];
 #pragma unroll
-        for (int j = 0; j < W; ++j) Q[j] = (j >= wi) ? (S[j] & r[j]) : 0ull;
-        to_sgpr<W>(Q);
+        for (int j = 0; j < W; ++j) Q[j] = (j >= wi) ? (S[j] & uniform_word(r[j])) : 0ull;
         if (popc_set<W>(Q) < (uint32_t)(J - 1)) continue;
         total += wave_cliques<W, J - 1>(Q, rows, lane);
       }
