@@ -424,23 +424,35 @@ kcl_status kcl_trim_write(const kcl_trim *t, const char *ckdeg, const char *ckco
  * Sim-kClist++ rounds on the GPU (1, 2, 4, ... iterations), the top PAVA
  * block (kCDensestMem.c:329-364) as the candidate of density D = p/q, the
  * D-core by k-clique degree as the region that must contain every densest
- * subgraph, and a host max-flow over the region's cliques (Goldberg's
- * network, kCDensestMem.c:434-467) that either certifies D - returning the
- * maximal densest subgraph - or yields a denser candidate.  Rounds stop at
- * max_iter iterations; regions whose clique count exceeds
+ * subgraph.  The candidate is stable when the region equals it (the
+ * storage-free counterpart of CDF_CheckStability, kCDensestMem.c:366-403);
+ * a stable candidate is first tested by the reference's closed-form
+ * criterion A (CDF_CheckDensestGoldberg, :405-432) over the allocation
+ * V_T / (T(T+3)/2), then the region's cliques go to a host max-flow of
+ * Goldberg's network (criterion B, :434-467; from round T = 8 on also for
+ * candidates that are not stable) that either certifies D - returning the
+ * maximal densest subgraph - or yields a denser candidate.
+ * Rounds stop at max_iter iterations; regions whose clique count exceeds
  * max_network_cliques are not sent to the max-flow.  Without a certificate
  * the result is the densest subgraph found, with upper_bound = the largest
  * average of the top-i normalised rho values (rho_T / (T(T+3)/((T+1)(T+2)))
  * is a convex combination of clique-to-member allocations, so every set S
  * has cliques(S) <= sum over S of it).
- * stat_path (nullable): the reference's stat_exact file (rows per round,
- * then the final record).  nodes_out (nullable, room for n entries of the
- * graph): input ids of the subgraph, ascending.
+ * stat_path (nullable): the reference's stat_exact file: a row per round,
+ * "%u\t%u\t%.12f\t%ld\t%s" with Info NOT STABLE (the region is larger than
+ * the candidate), STABLE BUT NOT DENSEST (it equals the candidate and
+ * neither criterion certified it) or OVER BUDGET (the candidate alone holds
+ * more than max_network_cliques cliques: not tested; the reference, which
+ * stores every clique, has no such row), then the final record with
+ * Stopping Condition Goldberg or Max Flow.  nodes_out (nullable, room for n
+ * entries of the graph): input ids of the subgraph, ascending.
  * ------------------------------------------------------------------- */
 typedef struct kcl_exact_opts {
   uint32_t max_iter;             /* 0 = 65536 */
   uint32_t max_seconds;          /* wall-clock budget, 0 = none */
-  uint64_t max_network_cliques;  /* 0 = 20,000,000 */
+  uint64_t max_network_cliques;  /* 0 = 100,000,000 (about (20 k + 8) bytes of
+                                    host memory per clique; at most
+                                    (2^32 - 1) / k) */
   uint32_t schedule;             /* kcl_schedule of the Sim iterations */
   uint32_t reserved0;
   const kcl_cost_model *model;   /* for KCL_SCHED_LEARNED (nullable) */
@@ -459,6 +471,11 @@ typedef struct kcl_exact_result {
   double seconds;
   double upper_bound;      /* bound on the maximum density from the last
                               allocation (equals density when certified) */
+  uint32_t stop;           /* certified by: 1 criterion A (Goldberg),
+                              2 max-flow; 0 not certified */
+  uint32_t degree_passes;  /* k-clique degree passes of the region tests */
+  double host_seconds;     /* host time of the certificates (clique
+                              listing, max-flow, criterion A) */
 } kcl_exact_result;
 
 kcl_status kcl_exact_run(kcl_graph *g, uint32_t k, const kcl_exact_opts *opts,

@@ -107,7 +107,8 @@ class _ExactResult(C.Structure):
     _fields_ = [("certified", C.c_uint32), ("num_iter", C.c_uint32), ("n", C.c_uint32),
                 ("m", C.c_uint32), ("cliques", C.c_uint64), ("density", C.c_double),
                 ("max_flow_calls", C.c_uint32), ("region", C.c_uint32), ("seconds", C.c_double),
-                ("upper_bound", C.c_double)]
+                ("upper_bound", C.c_double), ("stop", C.c_uint32),
+                ("degree_passes", C.c_uint32), ("host_seconds", C.c_double)]
 
 
 class _Record(C.Structure):
@@ -791,5 +792,5 @@ def exact(graph: Graph, k: int, max_iter: int = 0, max_network_cliques: int = 0,
     _check(lib().kcl_exact_run(graph.handle, k, C.byref(o),
                                stat_path.encode() if stat_path else None, t0_unix, C.byref(r),
                                _ptr(nodes)))
-    res = {f: getattr(r, f) for f, _ in _ExactResult._fields_}
+    res = {f: getattr(r, f) for f, _ in _ExactResult._fields_ if f != "reserved"}
     return res, nodes[:r.n].copy()

@@ -59,7 +59,7 @@ def test_exact_brute_force(kcl, seed, k):
     _say("exact brute", seed, k, members)
     with kcl.Graph(pairs) as g:
         r, nodes = kcl.exact(g, k)
-    assert r["certified"] == 1
+    assert r["certified"] == 1 and r["stop"] in (1, 2)
     if bp == 0:
         return  # no k-clique: every set has density 0 (reference: the whole graph)
     assert nodes.tolist() == members
@@ -153,21 +153,53 @@ def test_exact_cli_stat_file(kcl, tmp_path):
     assert lines[0].startswith("[Number of Iterations]\t[Number of Nodes]")
     assert lines[-2].startswith("[Number of Iterations]\t[Stopping Condition]")
     f = lines[-1].split("\t")
-    assert f[1] == "Max Flow" and int(f[2]) == 12 and int(f[3]) == 66
+    # The planted K12 is stable (its region is itself) and a complete graph,
+    # so the reference's closed-form criterion A certifies it.
+    assert f[1] == "Goldberg" and int(f[2]) == 12 and int(f[3]) == 66
     assert f[4] == "%.12f" % (math.comb(12, 4) / 12)
 
 
-def test_exact_over_budget_reports_bounds(kcl):
+def test_exact_over_budget_reports_bounds(kcl, tmp_path):
     """With a network budget too small for any certificate the result is
     the densest subgraph found, uncertified, with a valid upper bound."""
     rng = np.random.default_rng(9)
     pairs, _ = gg.erdos_renyi(400, 2400, 9)
     pairs = gg.plant_clique(pairs, 400, 14, rng)
+    stat = str(tmp_path / "stat.txt")
     with kcl.Graph(pairs) as g:
-        r_small, _ = kcl.exact(g, 4, max_network_cliques=10, max_iter=8)
+        r_small, _ = kcl.exact(g, 4, max_network_cliques=10, max_iter=8, stat_path=stat)
         r_full, nodes = kcl.exact(g, 4)
-    assert r_small["certified"] == 0 and r_small["max_flow_calls"] == 0
+    assert r_small["certified"] == 0 and r_small["max_flow_calls"] == 0 and r_small["stop"] == 0
+    labels = [r.split("\t")[4] for r in open(stat).read().splitlines()[1:]]
+    assert len(labels) == 4 and "OVER BUDGET" in labels
+    assert set(labels) <= {"OVER BUDGET", "NOT STABLE", "STABLE BUT NOT DENSEST"}
     assert r_small["upper_bound"] >= r_small["density"] > 0
     assert r_full["certified"] == 1
     assert r_small["density"] <= r_full["density"] <= r_small["upper_bound"] + 1e-9
     assert r_full["n"] == len(nodes)
+
+
+def test_exact_planted_clique_by_criterion_a(kcl):
+    """The k = 4 workload (ER 20k/100k + planted K30): the K30 is its own
+    region and complete, so criterion A certifies it without a max-flow."""
+    pairs = gg.workload("er20k", seed=1)
+    with kcl.Graph(pairs) as g:
+        r, nodes = kcl.exact(g, 4)
+    assert r["certified"] == 1 and r["stop"] == 1 and r["max_flow_calls"] == 0
+    assert r["n"] == 30 and r["cliques"] == math.comb(30, 4) and r["m"] == 435
+
+
+def test_exact_community_by_max_flow(kcl, oracle):
+    """A 60-node community with p = 0.8 planted in a Chung-Lu graph, k = 5:
+    not a clique, so the certificate is the max-flow; the result's clique
+    count is checked against the oracle's."""
+    pairs, rng = gg.chung_lu(20000, 120000, 2.3, 300.0, 31)
+    pairs = gg.finalize(gg.plant_community(pairs, 20000, 60, 0.8, rng), rng)
+    with kcl.Graph(pairs) as g:
+        r, nodes = kcl.exact(g, 5)
+    assert r["certified"] == 1 and r["stop"] == 2 and r["max_flow_calls"] >= 1
+    og = oracle.dag(pairs)
+    mask = np.zeros(og.n, np.uint8)
+    mask[og.rank[nodes]] = 1
+    assert oracle.count_in_set(og, 5, mask) == r["cliques"] and len(nodes) == r["n"]
+    assert r["n"] <= 60 and r["cliques"] > 10 ** 5

@@ -296,3 +296,26 @@ def test_reference_csr_loader_matches_its_own_build():
     assert np.array_equal(ca, cb) and ca.sum() > 0
     ref.free(a)
     ref.free(b)
+
+
+@pytest.mark.parametrize("seed", range(5))
+def test_oracle_count_in_set_brute_force(oracle, seed):
+    """k-cliques inside a node subset (the exact pipeline's checker) against
+    every k-subset of the subset, on random 14-node graphs and masks."""
+    rng = np.random.default_rng(900 + seed)
+    n = 14
+    pairs = np.array([(a, b) for a in range(n) for b in range(a + 1, n)
+                      if rng.random() < 0.55], np.uint32)
+    pairs = gg.finalize(pairs, rng)
+    g = oracle.dag(pairs)
+    adj = np.zeros((g.n_in, g.n_in), bool)
+    for a, b in pairs.tolist():
+        adj[a, b] = adj[b, a] = True
+    for k in (2, 3, 4, 5):
+        for _ in range(4):
+            keep = rng.random(g.n_in) < 0.7
+            want = sum(1 for sub in itertools.combinations(np.nonzero(keep)[0].tolist(), k)
+                       if all(adj[a, b] for a, b in itertools.combinations(sub, 2)))
+            mask = np.zeros(g.n, np.uint8)
+            mask[g.rank[np.nonzero(keep)[0]]] = 1
+            assert oracle.count_in_set(g, k, mask) == want
