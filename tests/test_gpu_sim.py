@@ -118,7 +118,6 @@ _BIG_PAIRS = {}
 def _big_pairs(gd):
     key = (str(gd["workload"][0]), int(gd["seed"]), int(gd["scale_down"]))
     if key not in _BIG_PAIRS:
-        _BIG_PAIRS.clear()
         pairs = gg.workload(key[0], seed=key[1], scale_down=key[2])
         assert hashlib.sha256(pairs.tobytes()).hexdigest() == str(gd["pairs_sha"][0])
         _BIG_PAIRS[key] = pairs
