@@ -367,6 +367,49 @@ def cpu_baseline_exact(pairs_file, res: dict, k: int, workload: str, gg) -> dict
                       f"{float(res['host_seconds']):.2f}s host certificate"}
 
 
+def run_trim(args, kcl, gg):
+    """k-clique core trimming (kClistCoreTrim) of a workload: (k-1)-core,
+    k-clique degrees on the GPU, the peel with heavy pops on the GPU; wall
+    clock from the edges in memory to the trimmed set.  cpu_baseline: the
+    reference's own kClistCoreTrim program (built from its sources) on the
+    same text file, single-threaded as it is written, with a time limit."""
+    import subprocess
+    wname, seed, sd, k, desc = WORKLOADS[args.workload]
+    pairs = workload_pairs(args, gg, wname, seed, sd)
+    log(f"bench trim: {args.workload} ({desc}), {pairs.shape[0]} edges")
+    t0 = time.perf_counter()
+    with kcl.Trim(pairs, k, gpu_peel_min_cliques=args.gpu_peel_min) as t:
+        elapsed = time.perf_counter() - t0
+        info = dict(t.info)
+    log(f"bench trim: {elapsed:.2f}s {info}")
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import REF_TRIM_BIN
+        with tempfile.TemporaryDirectory() as td:
+            path = os.path.join(td, "edges.txt")
+            gg.write_text(pairs, path)
+            c0 = time.perf_counter()
+            try:
+                r = subprocess.run([REF_TRIM_BIN, str(k), path, "d.txt", "c.txt", "e.txt", "b"],
+                                   cwd=td, capture_output=True, text=True,
+                                   timeout=args.cpu_timeout)
+                ok = r.returncode == 0
+            except subprocess.TimeoutExpired:
+                ok = False
+            cs = time.perf_counter() - c0
+        cpu = {"value": cs, "unit": "s", "cores": 1, "kind": "reference", "cpu_model": cpu_model(),
+               "sample": f"the reference kClistCoreTrim (one thread, as written) on the same edge "
+                         f"file: {'finished' if ok else 'stopped at the time limit'} after "
+                         f"{cs:.1f}s" + ("" if ok else " (a lower bound)")}
+        log(f"bench trim: reference {cs:.1f}s ({'finished' if ok else 'time limit'})")
+    out = {"metric": f"k-clique core trimming time (k={k})", "value": elapsed, "unit": "s",
+           "n_gpus": 1, "higher_is_better": False, "vs_baseline": None, "dtype": "u64",
+           "data": "synthetic (seeded generator, kclistpp_amd/graphgen.py)",
+           "config": {"workload": f"{args.workload}: {desc}", "k": k}, "result": info,
+           "cpu_baseline": cpu}
+    print(json.dumps(out), flush=True)
+
+
 def run_exact(args, kcl, gg):
     """Time-to-exact: text edge file -> graph -> certified maximal densest
     subgraph (kcl_exact_run), wall clock."""
@@ -432,8 +475,14 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--exact-budget", type=int, default=240,
                     help="exact mode: seconds before giving up on a certificate")
-    ap.add_argument("--mode", default="sim", choices=["sim", "exact"],
-                    help="sim: visits/s of one iteration (default); exact: time-to-exact")
+    ap.add_argument("--mode", default="sim", choices=["sim", "exact", "trim"],
+                    help="sim: visits/s of one iteration (default); exact: time-to-exact; "
+                         "trim: k-clique core trimming time")
+    ap.add_argument("--gpu-peel-min", type=int, default=0,
+                    help="trim mode: pops through at least this many cliques go to the GPU "
+                         "(0: the library's default)")
+    ap.add_argument("--cpu-timeout", type=float, default=900.0,
+                    help="trim mode: time limit of the reference program")
     ap.add_argument("--plain-steps", type=int, default=2,
                     help="iterations of the plain (d+^2) schedule timed on the main run's "
                          "states (0: skip)")
@@ -459,6 +508,9 @@ def main():
 
     if args.mode == "exact":
         run_exact(args, kcl, gg)
+        return
+    if args.mode == "trim":
+        run_trim(args, kcl, gg)
         return
     if args.workload == "batch512":
         run_batch(args, kcl, gg)

@@ -397,13 +397,26 @@ typedef struct kcl_trim_info {
   uint32_t ntrim;   /* nodes kept by the trim */
   uint64_t ncktrim; /* k-cliques of the trimmed graph */
   uint32_t mtrim;   /* edges of the trimmed graph */
-  uint32_t reserved;
+  uint32_t gpu_pops; /* peel pops whose cliques were counted on the GPU */
   double seconds_degrees; /* core + relabel + GPU k-clique degrees */
-  double seconds_peel;    /* host peel */
+  double seconds_peel;    /* the peel (host heap, heavy pops on the GPU) */
 } kcl_trim_info;
 
 kcl_status kcl_trim_create(const uint32_t *pairs, uint64_t m, uint32_t n_in, uint32_t k,
                            kcl_trim **out);
+/* The same with options (no counterpart in the reference, whose peel is one
+ * host thread).  The peel's heap stays on the host (its tie order defines
+ * ckcore, ckdens and the trimmed set); a popped node that still lies in at
+ * least gpu_peel_min_cliques alive k-cliques (0 = 2^16) has them counted
+ * on the GPU - its alive neighbours as one root's members, every member
+ * credited - instead of by the host's enumeration (oneshortpass,
+ * kClistCoreTrim.c:577-603).  Results do not depend on it. */
+typedef struct kcl_trim_opts {
+  uint64_t gpu_peel_min_cliques;
+  uint32_t reserved[4];
+} kcl_trim_opts;
+kcl_status kcl_trim_create_opts(const uint32_t *pairs, uint64_t m, uint32_t n_in, uint32_t k,
+                                const kcl_trim_opts *opts, kcl_trim **out);
 void kcl_trim_destroy(kcl_trim *t);
 kcl_status kcl_trim_get_info(const kcl_trim *t, kcl_trim_info *info);
 /* Per new id (n2 entries each, every pointer nullable): input id, k-clique

@@ -80,7 +80,7 @@ class _TrimInfo(C.Structure):
                 ("core", C.c_uint32), ("ncktot", C.c_uint64), ("ckcore", C.c_uint64),
                 ("ckrho", C.c_double), ("rho", C.c_double), ("size", C.c_uint32),
                 ("ntrim", C.c_uint32), ("ncktrim", C.c_uint64), ("mtrim", C.c_uint32),
-                ("reserved", C.c_uint32), ("seconds_degrees", C.c_double),
+                ("gpu_pops", C.c_uint32), ("seconds_degrees", C.c_double),
                 ("seconds_peel", C.c_double)]
 
 
@@ -95,6 +95,10 @@ class _BatchInfo(C.Structure):
                 ("edges", C.c_uint64), ("cliques", C.c_uint64), ("tasks", C.c_uint32),
                 ("row_bits", C.c_uint32), ("per_cu", C.c_uint32), ("reserved", C.c_uint32),
                 ("plan_ms", C.c_double), ("tune_ms", C.c_double)]
+
+
+class _TrimOpts(C.Structure):
+    _fields_ = [("gpu_peel_min_cliques", C.c_uint64), ("reserved", C.c_uint32 * 4)]
 
 
 class _ExactOpts(C.Structure):
@@ -133,7 +137,8 @@ EXPORTS = [
     "kcl_clique_enum_noalloc", "kcl_sim_create",
     "kcl_sim_destroy", "kcl_sim_step", "kcl_sim_counts", "kcl_sim_rho", "kcl_sim_set_rho",
     "kcl_sim_totals", "kcl_sim_last_pass", "kcl_sim_plan_info", "kcl_sim_root_cliques", "kcl_sim_profile_pass", "kcl_sim_find_densest", "kcl_cmpsync_run",
-    "kcl_device_get_info", "kcl_version", "kcl_trim_create", "kcl_trim_destroy",
+    "kcl_device_get_info", "kcl_version", "kcl_trim_create", "kcl_trim_create_opts",
+    "kcl_trim_destroy",
     "kcl_trim_get_info", "kcl_trim_nodes", "kcl_trim_edges", "kcl_trim_write", "kcl_exact_run",
     "kcl_cost_model_create", "kcl_cost_model_destroy", "kcl_cost_model_load",
     "kcl_cost_model_save", "kcl_cost_model_get", "kcl_cost_model_set", "kcl_cost_model_predict",
@@ -191,6 +196,8 @@ def lib() -> C.CDLL:
                              C.POINTER(_Record), C.POINTER(u32)], C.c_int),
         "kcl_device_get_info": ([C.POINTER(_DeviceInfo)], C.c_int),
         "kcl_trim_create": ([vp, u64, u32, u32, C.POINTER(vp)], C.c_int),
+        "kcl_trim_create_opts": ([vp, u64, u32, u32, C.POINTER(_TrimOpts), C.POINTER(vp)],
+                                 C.c_int),
         "kcl_trim_destroy": ([vp], None),
         "kcl_trim_get_info": ([vp, C.POINTER(_TrimInfo)], C.c_int),
         "kcl_trim_nodes": ([vp, vp, vp, vp, vp, vp], C.c_int),
@@ -724,11 +731,16 @@ class Trim:
     """k-clique core trimming (kClistCoreTrim): k-clique degrees on the GPU,
     the reference's heaps on the host."""
 
-    def __init__(self, pairs: np.ndarray, k: int, n_in: int = 0):
+    def __init__(self, pairs: np.ndarray, k: int, n_in: int = 0, gpu_peel_min_cliques: int = 0):
+        """gpu_peel_min_cliques: peel pops whose node lies in at least this
+        many alive k-cliques are counted on the GPU (0 = the library's
+        default); results do not depend on it."""
         pairs = np.ascontiguousarray(pairs, dtype=np.uint32).reshape(-1, 2)
         h = C.c_void_p()
-        _check(lib().kcl_trim_create(_ptr(pairs) if pairs.size else None, pairs.shape[0], n_in,
-                                     k, C.byref(h)))
+        o = _TrimOpts()
+        o.gpu_peel_min_cliques = gpu_peel_min_cliques
+        _check(lib().kcl_trim_create_opts(_ptr(pairs) if pairs.size else None, pairs.shape[0],
+                                          n_in, k, C.byref(o), C.byref(h)))
         self._h = h
         self.k = k
         info = _TrimInfo()

@@ -167,6 +167,10 @@ struct PassLaunch {
   const uint32_t *eclass_count = nullptr;
   const uint32_t *light_from = nullptr;
   uint32_t max_per_cu = 0;
+  // Vertex tasks take their members from this device list (ascending ids,
+  // at most the class's 64 W) instead of N+(s); nullptr: N+(s).
+  const uint32_t *list = nullptr;
+  uint32_t nlist = 0;
 };
 kcl_status launch_pass_raw(const kcl_graph *g, const uint64_t *d_key, uint32_t k, PassMode mode,
                            const PassLaunch &plan, unsigned long long *d_cnt, hipStream_t stream,

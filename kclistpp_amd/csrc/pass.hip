@@ -124,7 +124,7 @@ kcl_status launch_pass_raw(const kcl_graph *g, const uint64_t *d_key, uint32_t k
                          g->core_h, plan.tasks + begin[c], count,
                          grp ? count : std::min(plan.light_from[c], count), g->dense_items,
                          plan.tickets + 2 * (grp * kNumClasses + c), plan.tickets + kTickets,
-                         (uint32_t)mode, d_cnt,
+                         plan.list, plan.nlist, (uint32_t)mode, d_cnt,
                          prof ? prof + 2 * (size_t)begin[c] : nullptr);
       KCL_HIP(hipGetLastError());
     }

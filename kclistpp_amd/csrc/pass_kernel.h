@@ -335,6 +335,8 @@ __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restric
                                                        float dense_items,
                                                        uint32_t *__restrict__ ticket,
                                                        uint32_t *__restrict__ status,
+                                                       const uint32_t *__restrict__ list,
+                                                       uint32_t nlist,
                                                        uint32_t mode,
                                                        unsigned long long *__restrict__ cnt,
                                                        unsigned long long *__restrict__ prof) {
@@ -404,12 +406,15 @@ __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restric
     uint64_t ks;
     uint32_t eu = s;  // edge tasks: the second root node u
     if (!edge_task) {
-      const uint32_t b0 = cd[s];
-      d = cd[s + 1] - b0;
+      // Members: N+(s), or the launch's explicit list (ascending ids, at
+      // most 64 W: the k-clique-core peel's alive neighbours of s).
+      const uint32_t b0 = list ? 0u : cd[s];
+      const uint32_t *mlist = list ? list : adj + b0;
+      d = list ? nlist : cd[s + 1] - b0;
       ks = key[s];
       // Stage the members, their keys and adjacency extents.
       for (uint32_t i = tid; i < d; i += NT) {
-        const uint32_t x = adj[b0 + i];
+        const uint32_t x = mlist[i];
         sh.node[i] = x;
         sh.key[i] = key[x];
         const uint32_t xb = cd[x];
@@ -889,8 +894,8 @@ __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restric
 
 using PassFn = void (*)(const uint32_t *, const uint32_t *, const uint64_t *,
                         const unsigned long long *, uint32_t, const Task *, uint32_t, uint32_t,
-                        float, uint32_t *, uint32_t *, uint32_t, unsigned long long *,
-                        unsigned long long *);
+                        float, uint32_t *, uint32_t *, const uint32_t *, uint32_t, uint32_t,
+                        unsigned long long *, unsigned long long *);
 
 constexpr int kMaxTemplK = 10;  // k handled by compiled kernels: 2..kMaxTemplK
 

@@ -9,7 +9,12 @@
 //      the enumeration pass with every member credited
 //   4. k-clique core peel: min-heap on the k-clique degrees, each pop
 //      removes the cliques through the popped node from its alive
-//      neighbours (kcliquecore + oneshortpass, :577-666)     host, sequential
+//      neighbours (kcliquecore + oneshortpass, :577-666).  The heap is the
+//      host's (its tie order defines the results); a pop whose node still
+//      lies in at least gpu_peel_min_cliques cliques counts them on the GPU
+//      instead: one root (the popped node, members its alive neighbours,
+//      split into parts that fill the device) with every member credited,
+//      then a gather of the neighbours' counts
 //   5. outputs (printckdeg, printdensest, :419-429, :668-705)
 // Steps 1 and 4 define the results through their heaps' tie order, so they
 // replay the reference's heap discipline on the host.
@@ -24,6 +29,7 @@
 
 struct kcl_trim {
   kcl_trim_info info{};
+  kcl_trim_opts opts{};
   uint32_t k = 0;
   std::vector<uint32_t> map;     // new id -> input id (n2)
   std::vector<uint64_t> nck;     // k-clique degree per new id (n2)
@@ -36,6 +42,10 @@ struct kcl_trim {
 namespace kcl {
 
 namespace {
+
+// Pops whose node lies in at least this many alive k-cliques count them on
+// the GPU (below it the host's enumeration is faster than a launch).
+constexpr uint64_t kDefaultGpuPeelCliques = 1ull << 16;
 
 // Binary min-heap with decrease-by-delta, the reference's tie discipline
 // (kClistCoreTrim.c:64-133 and :433-533): inserted in key order, sift-up
@@ -142,6 +152,88 @@ void peel_descend(const std::vector<uint32_t> &cd, const std::vector<uint32_t> &
 
 }  // namespace
 
+// out[i] = cnt[list[i]], out[n] = cnt[root]; those counters back to zero.
+__global__ void gather_clear_kernel(const uint32_t *__restrict__ list, uint32_t n, uint32_t root,
+                                    unsigned long long *__restrict__ cnt,
+                                    unsigned long long *__restrict__ out) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i <= n; i += gridDim.x * blockDim.x) {
+    const uint32_t v = i < n ? list[i] : root;
+    out[i] = cnt[v];
+    cnt[v] = 0;
+  }
+}
+
+// One pop's oneshortpass on the GPU: the k-cliques through u among its
+// alive neighbours M (ascending, |M| <= kMaxRootDegree), every member
+// credited; nck2[M[i]] += cliques through u and M[i].
+struct GpuPeel {
+  kcl_graph *g = nullptr;
+  uint32_t k = 0;
+  const unsigned long long *key = nullptr;  // zeros (credit-all ignores keys)
+  unsigned long long *cnt = nullptr;        // n2 counters, zero between pops
+  DevBuf<uint32_t> list, tickets;
+  DevBuf<Task> tasks;
+  DevBuf<unsigned long long> out;
+  std::vector<Task> host_tasks;
+  std::vector<unsigned long long> host_out;
+  uint32_t pops = 0;
+
+  kcl_status init() {
+    KCL_TRY(list.alloc(kMaxRootDegree));
+    KCL_TRY(tickets.alloc(kPassTicketWords));
+    KCL_TRY(tasks.alloc(kMaxSplit));
+    KCL_TRY(out.alloc(kMaxRootDegree + 1));
+    host_out.resize(kMaxRootDegree + 1);
+    return KCL_OK;
+  }
+
+  static constexpr uint32_t kMaxSplit = 4096;
+
+  kcl_status pop(uint32_t u, const uint32_t *m, uint32_t d, uint64_t cliques,
+                 uint64_t *nck2) {
+    hipStream_t st = g->stream;
+    uint32_t c = 0;
+    while (kClassWords[c] * 64 < d) ++c;
+    // Parts of about 2^20 cliques, within the items a part can take.
+    const uint64_t items = k >= 5 ? (uint64_t)d * (d - 1) / 2 : d;
+    uint64_t np = cliques >> 20;
+    np = std::max<uint64_t>(1, std::min<uint64_t>({np, items, (uint64_t)kMaxSplit}));
+    host_tasks.resize(np);
+    for (uint32_t p = 0; p < np; ++p) host_tasks[p] = Task{u, p | ((uint32_t)np << 16)};
+    KCL_HIP(hipMemcpyAsync(tasks.p, host_tasks.data(), np * sizeof(Task), hipMemcpyHostToDevice,
+                           st));
+    KCL_HIP(hipMemcpyAsync(list.p, m, (size_t)d * 4, hipMemcpyHostToDevice, st));
+    uint32_t begin[kNumClasses + 1] = {}, count[kNumClasses] = {}, light[kNumClasses] = {};
+    const uint32_t zero[kNumClasses + 1] = {};
+    for (uint32_t x = c + 1; x <= kNumClasses; ++x) begin[x] = (uint32_t)np;
+    count[c] = (uint32_t)np;
+    light[c] = (uint32_t)np;
+    PassLaunch pl;
+    pl.tasks = tasks.p;
+    pl.tickets = tickets.p;
+    pl.class_begin = begin;
+    pl.class_count = count;
+    pl.eclass_begin = zero;
+    pl.eclass_count = zero;
+    pl.light_from = light;
+    pl.list = list.p;
+    pl.nlist = d;
+    KCL_TRY(launch_pass_raw(g, (const uint64_t *)key, k, kCreditAll, pl, cnt, st));
+    gather_clear_kernel<<<(d + 256) / 256, 256, 0, st>>>(list.p, d, u, cnt, out.p);
+    KCL_HIP(hipGetLastError());
+    uint32_t status = 0;
+    KCL_HIP(hipMemcpyAsync(host_out.data(), out.p, ((size_t)d + 1) * 8, hipMemcpyDeviceToHost,
+                           st));
+    KCL_HIP(hipMemcpyAsync(&status, tickets.p + kPassTicketWords - 1, 4, hipMemcpyDeviceToHost,
+                           st));
+    KCL_HIP(hipStreamSynchronize(st));
+    KCL_TRY(pass_status_error(status));
+    for (uint32_t i = 0; i < d; ++i) nck2[m[i]] += host_out[i];
+    ++pops;
+    return KCL_OK;
+  }
+};
+
 kcl_status trim_run(kcl_trim *T, const uint32_t *pairs, uint64_t m, uint32_t n_in) {
   const uint32_t kmax = T->k;
   kcl_trim_info &info = T->info;
@@ -221,9 +313,9 @@ kcl_status trim_run(kcl_trim *T, const uint32_t *pairs, uint64_t m, uint32_t n_i
     return fail(KCL_E_UNSUPPORTED, "k = %u: the enumerator is compiled for k <= %u", kmax,
                 max_pass_k());
   }
+  DevBuf<unsigned long long> cnt, key;  // also the peel's counters (zeroed)
   if (g->n > 0 && kmax <= g->core + 1) {
     DevicePlan dp;
-    DevBuf<unsigned long long> cnt, key;
     PassPlan p;
     do {
       if ((st = make_plan(g, kmax, 1u << 17, g->stream, &p, nullptr, nullptr)) != KCL_OK) break;
@@ -243,12 +335,15 @@ kcl_status trim_run(kcl_trim *T, const uint32_t *pairs, uint64_t m, uint32_t n_i
       if (e == hipSuccess) e = hipStreamSynchronize(g->stream);
       if (e != hipSuccess) st = fail(KCL_E_HIP, "hipMemcpy: %s", hipGetErrorString(e));
       else st = pass_status(dp, g->stream);
+      if (st == KCL_OK) st = zero_u64(cnt.p, g->n, g->stream);
     } while (0);
   }  // else no k-clique can exist: the degrees stay zero
   std::vector<uint32_t> cd = g->cd, adj = g->adj;  // DAG rows, ascending
   cd.resize((size_t)n2 + 1, cd.empty() ? 0u : cd.back());
-  kcl_graph_destroy(g);
-  if (st != KCL_OK) return st;
+  if (st != KCL_OK) {
+    kcl_graph_destroy(g);
+    return st;
+  }
   uint64_t tot = 0;
   for (uint32_t v = 0; v < n2; ++v) tot += T->nck[v];
   tot /= kmax;
@@ -280,6 +375,21 @@ kcl_status trim_run(kcl_trim *T, const uint32_t *pairs, uint64_t m, uint32_t n_i
   std::vector<uint32_t> alive_nbr(maxdeg + 1), first(maxdeg + 1);
   std::vector<std::vector<uint32_t>> buf(kmax + 1, std::vector<uint32_t>(maxdeg + 1));
   std::vector<uint32_t> ck(kmax + 1);
+  // Heavy pops on the GPU (the degrees' graph, counters and zero keys).
+  GpuPeel gpu;
+  const bool gpu_ok = cnt.p && kmax <= max_pass_k();
+  if (gpu_ok) {
+    gpu.g = g;
+    gpu.k = kmax;
+    gpu.key = key.p;
+    gpu.cnt = cnt.p;
+    if ((st = gpu.init()) != KCL_OK) {
+      kcl_graph_destroy(g);
+      return st;
+    }
+  }
+  const uint64_t gpu_min = T->opts.gpu_peel_min_cliques ? T->opts.gpu_peel_min_cliques
+                                                        : kDefaultGpuPeelCliques;
   double ckrho = (double)ncktot / (double)n2;
   uint32_t size = n2;
   // 32-bit products as in the reference (they wrap for n2 > 65536).
@@ -295,7 +405,12 @@ kcl_status trim_run(kcl_trim *T, const uint32_t *pairs, uint64_t m, uint32_t n_i
     uint32_t s = 0;
     for (uint32_t j = cd0[u]; j < cd0[u + 1]; ++j)
       if (h.alive(adj0[j])) alive_nbr[s++] = adj0[j];
-    if (s > kmax - 2) {
+    if (s > kmax - 2 && gpu_ok && kv.val >= gpu_min && s <= kMaxRootDegree) {
+      if ((st = gpu.pop(u, alive_nbr.data(), s, kv.val, nck2.data())) != KCL_OK) {
+        kcl_graph_destroy(g);
+        return st;
+      }
+    } else if (s > kmax - 2) {
       ck[0] = u;
       for (uint32_t j = 0; j < s; ++j) {
         const uint32_t v = alive_nbr[j];
@@ -339,6 +454,8 @@ kcl_status trim_run(kcl_trim *T, const uint32_t *pairs, uint64_t m, uint32_t n_i
     T->ic_key[i] = kv.key;
     T->ic_val[i] = c;
   }
+  kcl_graph_destroy(g);
+  info.gpu_pops = gpu.pops;
   info.ckcore = c;
   info.ckrho = ckrho;
   info.rho = erho;
@@ -361,6 +478,12 @@ using kcl::fail;
 
 extern "C" kcl_status kcl_trim_create(const uint32_t *pairs, uint64_t m, uint32_t n_in,
                                       uint32_t k, kcl_trim **out) {
+  return kcl_trim_create_opts(pairs, m, n_in, k, nullptr, out);
+}
+
+extern "C" kcl_status kcl_trim_create_opts(const uint32_t *pairs, uint64_t m, uint32_t n_in,
+                                           uint32_t k, const kcl_trim_opts *opts,
+                                           kcl_trim **out) {
   if (!out) return fail(KCL_E_INVALID_ARG, "out is NULL");
   *out = nullptr;
   if (k < 3 || k > 255)
@@ -380,6 +503,7 @@ extern "C" kcl_status kcl_trim_create(const uint32_t *pairs, uint64_t m, uint32_
   kcl_trim *T = new (std::nothrow) kcl_trim();
   if (!T) return fail(KCL_E_OOM_HOST, "out of host memory");
   T->k = k;
+  if (opts) T->opts = *opts;
   kcl_status s;
   try {
     s = kcl::trim_run(T, pairs, m, n_in);

@@ -27,12 +27,17 @@ def _dens_line(info, mp, pk):
     return "%f %f %u" % (info["ckrho"], info["rho"], size) + "".join(f" {v}" for v in ids)
 
 
+@pytest.mark.parametrize("gpu_min", [0, 1])
 @pytest.mark.parametrize("case", TRIM)
-def test_trim_matches_reference(kcl, case):
+def test_trim_matches_reference(kcl, case, gpu_min):
+    """gpu_min = 1: every peel pop through at least one clique counts its
+    cliques on the GPU (the default sends only heavy pops there)."""
     gd = load_golden(case)
     k = int(gd["k"])
-    _say("trim", case)
-    with kcl.Trim(golden_pairs(gd), k) as t:
+    _say("trim", case, "gpu_min", gpu_min)
+    with kcl.Trim(golden_pairs(gd), k, gpu_peel_min_cliques=gpu_min) as t:
+        if gpu_min == 1 and t.info["ncktot"] > 0:
+            assert t.info["gpu_pops"] > 0
         mp, nck, pk, pc, kept = t.nodes()
         assert np.array_equal(np.stack([mp, nck]).T.astype(np.uint64), gd["ckdeg"])
         assert np.array_equal(np.stack([mp[pk], pc]).T.astype(np.uint64), gd["ckcore"])
@@ -63,13 +68,14 @@ def test_trim_cli_files(kcl, case, tmp_path):
     assert [int(x) for x in stat[1].split("\t")[:3]] == gd["stat"].tolist()
 
 
+@pytest.mark.parametrize("gpu_min", [0, 1, 1000])
 @pytest.mark.parametrize("k", [3, 4, 5, 6, 7])
-def test_trim_vs_oracle(kcl, oracle, k):
+def test_trim_vs_oracle(kcl, oracle, k, gpu_min):
     pairs, rng = gg.chung_lu(2500, 20000, 2.2, 250.0, 40 + k)
     pairs = gg.finalize(gg.plant_community(pairs, 2500, 36, 0.85, rng), rng)
     info_o, mp_o, nck_o, pk_o, pc_o, trimmed_o = oracle.trim(pairs, k)
-    _say("trim vs oracle k", k)
-    with kcl.Trim(pairs, k) as t:
+    _say("trim vs oracle k", k, "gpu_min", gpu_min)
+    with kcl.Trim(pairs, k, gpu_peel_min_cliques=gpu_min) as t:
         mp, nck, pk, pc, kept = t.nodes()
         assert np.array_equal(mp, mp_o) and np.array_equal(nck, nck_o)
         assert np.array_equal(pk, pk_o) and np.array_equal(pc, pc_o)
