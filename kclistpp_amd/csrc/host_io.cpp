@@ -9,6 +9,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <thread>
+#include <vector>
 
 #include "kcl_internal.h"
 
@@ -79,6 +81,92 @@ bool scan_u32(const char *&p, const char *end, uint32_t *out) {
   return true;
 }
 
+// The text of an edge list parsed by the host threads: the file is cut
+// into chunks at white space (so no number straddles two chunks); each
+// thread counts its chunk's numbers up to its first text that does not
+// parse, the counts are summed in order to give every chunk its first
+// number's index and the index where parsing stops (the first such text in
+// the file, like fscanf), then each thread converts its chunk into place.
+// Pairs: floor(numbers / 2), the reference's "%u %u" loop.
+kcl_status parse_text(const char *base, size_t size, const char *path, uint32_t **out,
+                      uint64_t *pairs_out, uint32_t *max_out) {
+  const uint32_t hw = std::thread::hardware_concurrency();
+  const uint32_t W = size < (1u << 22) ? 1u : std::max(1u, std::min(16u, hw ? hw : 1u));
+  std::vector<const char *> cut(W + 1);
+  cut[0] = base;
+  cut[W] = base + size;
+  for (uint32_t t = 1; t < W; ++t) {
+    const char *c = std::max(cut[t - 1], base + size * t / W);
+    while (c < base + size && !isspace((unsigned char)*c)) ++c;
+    cut[t] = c;
+  }
+  struct Part {
+    uint64_t numbers = 0;
+    bool stopped = false;  // text that does not parse inside this chunk
+    uint32_t mx = 0;
+  };
+  std::vector<Part> part(W);
+  // Numbers of chunk t (into dst[0, limit) when dst is set).  After a
+  // number, white space or a sign may follow (fscanf's next "%u" starts
+  // there); any other text ends the input after that number.
+  auto scan_chunk = [&](uint32_t t, uint32_t *dst, uint64_t limit) {
+    const char *p = cut[t], *end = cut[t + 1];
+    Part &pt = part[t];
+    uint64_t n = 0;
+    for (;;) {
+      while (p < end && isspace((unsigned char)*p)) ++p;
+      if (p >= end) break;
+      uint32_t v;
+      if (!scan_u32(p, end, &v)) {
+        pt.stopped = true;
+        break;
+      }
+      if (dst) {
+        if (n >= limit) break;
+        dst[n] = v;
+        pt.mx = std::max(pt.mx, v);
+      }
+      ++n;
+      if (p < end && !isspace((unsigned char)*p) && *p != '+' && *p != '-') {
+        pt.stopped = true;
+        break;
+      }
+    }
+    if (!dst) pt.numbers = n;
+  };
+  auto run = [&](auto fn) {
+    std::vector<std::thread> th;
+    for (uint32_t t = 1; t < W; ++t) th.emplace_back(fn, t);
+    fn(0);
+    for (auto &x : th) x.join();
+  };
+  run([&](uint32_t t) { scan_chunk(t, nullptr, 0); });
+  std::vector<uint64_t> first(W + 1, 0);
+  uint32_t last = W;
+  for (uint32_t t = 0; t < W; ++t) {
+    first[t + 1] = first[t] + part[t].numbers;
+    if (part[t].stopped) {
+      last = t + 1;
+      break;
+    }
+  }
+  const uint64_t numbers = first[last];
+  const uint64_t pairs = numbers / 2;
+  uint32_t *buf = (uint32_t *)malloc(std::max<uint64_t>(2 * pairs, 2) * sizeof(uint32_t));
+  if (!buf) return kcl::fail(KCL_E_OOM_HOST, "out of host memory reading %s", path);
+  for (auto &pt : part) pt.mx = 0;
+  run([&](uint32_t t) {
+    if (t >= last || first[t] >= 2 * pairs) return;
+    scan_chunk(t, buf + first[t], 2 * pairs - first[t]);
+  });
+  uint32_t mx = 0;
+  for (auto &pt : part) mx = std::max(mx, pt.mx);
+  *out = buf;
+  *pairs_out = pairs;
+  *max_out = mx;
+  return KCL_OK;
+}
+
 bool ends_with(const char *s, const char *suf) {
   size_t a = strlen(s), b = strlen(suf);
   return a >= b && strcmp(s + a - b, suf) == 0;
@@ -131,29 +219,7 @@ extern "C" kcl_status kcl_read_edge_list(const char *path, uint32_t **pairs, uin
       }
     }
   } else {
-    uint64_t cap = size / 4 + 16;  // at least 4 bytes per number: "a b\n"
-    buf = (uint32_t *)malloc(cap * sizeof(uint32_t));
-    if (!buf) rc = fail(KCL_E_OOM_HOST, "out of host memory reading %s", path);
-    const char *p = base, *end = base + size;
-    while (rc == KCL_OK) {
-      uint32_t a, b;
-      if (!scan_u32(p, end, &a)) break;
-      if (!scan_u32(p, end, &b)) break;
-      if (2 * cnt + 2 > cap) {
-        cap = cap * 2 + 16;
-        uint32_t *nb = (uint32_t *)realloc(buf, cap * sizeof(uint32_t));
-        if (!nb) {
-          rc = fail(KCL_E_OOM_HOST, "out of host memory reading %s", path);
-          break;
-        }
-        buf = nb;
-      }
-      buf[2 * cnt] = a;
-      buf[2 * cnt + 1] = b;
-      mx = a > mx ? a : mx;
-      mx = b > mx ? b : mx;
-      ++cnt;
-    }
+    rc = parse_text(base, size, path, &buf, &cnt, &mx);
   }
   if (base) munmap((void *)base, size);
   if (rc != KCL_OK) {

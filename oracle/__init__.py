@@ -25,6 +25,7 @@ _u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
 _u64p = np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")
 _f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
 _u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+_u32p_raw = C.POINTER(C.c_uint32)
 
 
 class _Densest(C.Structure):
@@ -94,7 +95,24 @@ class Oracle:
         L.orc_count_in_set.argtypes = [C.c_uint32, _u32p, _u32p, C.c_uint64, _u32p, _u32p,
                                        C.c_uint32, _u8p]
         L.orc_count_in_set.restype = C.c_uint64
+        L.orc_read_edge_list.argtypes = [C.c_char_p, C.POINTER(_u32p_raw), C.POINTER(C.c_uint64),
+                                         C.POINTER(C.c_uint32)]
+        L.orc_free.argtypes = [C.c_void_p]
         self.L = L
+
+    # -- input -----------------------------------------------------------
+    def read_edge_list(self, path: str) -> tuple[np.ndarray, int]:
+        """fscanf("%u %u") over the whole file (Graph.c:34-53): pairs, max id + 1."""
+        p = _u32p_raw()
+        m, n = C.c_uint64(0), C.c_uint32(0)
+        if self.L.orc_read_edge_list(path.encode(), C.byref(p), C.byref(m), C.byref(n)) != 0:
+            raise OSError(path)
+        try:
+            out = np.ctypeslib.as_array(p, shape=(2 * m.value,)).copy() if m.value \
+                else np.zeros(0, np.uint32)
+        finally:
+            self.L.orc_free(p)
+        return out.reshape(-1, 2), int(n.value)
 
     # -- graph ---------------------------------------------------------
     def rank(self, pairs: np.ndarray, n: int | None = None) -> np.ndarray:

@@ -98,6 +98,44 @@ def test_reader_matches_oracle_reader(kcl, oracle, tmp_path):
     assert n_in == int(pairs.max()) + 1
 
 
+def _fscanf_text(rng, tokens, stop_at=None):
+    """Random numbers joined by assorted white space, with signed and glued
+    forms; a text that does not parse at token index `stop_at`."""
+    out = []
+    for i in range(tokens):
+        r = rng.random()
+        if stop_at is not None and i == stop_at:
+            t = str(int(rng.integers(0, 9))) + "q"
+        elif r < 0.02:
+            t = "-" + str(int(rng.integers(2, 100)))
+        elif r < 0.03:
+            t = str(int(rng.integers(0, 9))) + "-" + str(int(rng.integers(2, 9)))
+        elif r < 0.04:
+            t = "+" + str(int(rng.integers(0, 9)))
+        else:
+            t = str(int(rng.integers(0, 10**7)))
+        out.append(t + (" ", "\n", "\t", "  ", " \r\n")[int(rng.integers(0, 5))])
+    return "".join(out)
+
+
+@pytest.mark.parametrize("stop", [None, 0.37, 0.999])
+def test_reader_threaded_matches_fscanf(kcl, oracle, tmp_path, stop):
+    # > 4 MiB of text: the reader cuts it into chunks parsed by several
+    # threads; the result must equal the fscanf("%u %u") loop, including
+    # where text that does not parse sits deep inside one chunk.
+    rng = np.random.default_rng(11)
+    tokens = 900_001
+    text = _fscanf_text(rng, tokens, None if stop is None else int(stop * tokens))
+    assert len(text) > (1 << 22)
+    path = str(tmp_path / "big.txt")
+    with open(path, "w") as f:
+        f.write(text)
+    got, n_in = kcl.read_edge_list(path)
+    want, n_want = oracle.read_edge_list(path)
+    assert np.array_equal(got, want)
+    assert n_in == n_want
+
+
 CLI = os.path.join(ROOT, "kclistpp_amd", "bin", "kCDensestCmpSync")
 
 
