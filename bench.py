@@ -410,6 +410,93 @@ def run_trim(args, kcl, gg):
     print(json.dumps(out), flush=True)
 
 
+def makespan_of(sim, clock_hz: float, cus: int) -> dict:
+    """One profiled pass of sim's plan at its current rho: per-task
+    shader-clock cycles priced against the GPU's resident workgroups.
+
+    A class-c workgroup holds 1/per_cu_c of a CU for its task's cycles, so
+    the pass needs at least sum_t cycles_t / per_cu_c CU-cycles; spread over
+    every CU at the peak shader clock that is the bound below which no order
+    or packing of these tasks can finish (the timers count shader cycles,
+    whose rate is at most the peak, so the bound errs low)."""
+    prof = sim.profile_pass()
+    cyc = prof["total_cycles"].astype(np.float64)
+    cu_cycles, classes = 0.0, []
+    for c in sim.plan_classes():
+        seg = cyc[c["first"]:c["first"] + c["count"]]
+        work = float(seg.sum())
+        cu_cycles += work / c["per_cu"]
+        classes.append({"words": c["words"], "k": c["k"], "edge": c["edge"], "tasks": c["count"],
+                        "waves": c["waves"], "per_cu": c["per_cu"], "blocks": c["blocks"],
+                        "work_cu_ms": round(work / c["per_cu"] / clock_hz * 1e3, 3),
+                        "alone_lb_ms": round(work / max(c["blocks"], 1) / clock_hz * 1e3, 3),
+                        "max_task_ms": round(float(seg.max()) / clock_hz * 1e3, 3)})
+    return {"tasks": int(cyc.size), "lb_ms": cu_cycles / cus / clock_hz * 1e3,
+            "max_task_ms": float(cyc.max()) / clock_hz * 1e3 if cyc.size else 0.0,
+            "build_share": float(prof["build_cycles"].sum()) / max(float(cyc.sum()), 1.0),
+            "classes": classes}
+
+
+def run_makespan(args, kcl, gg):
+    """Pass time of each schedule against a makespan lower bound, on the
+    same rho state (after --warmup iterations of the learned schedule).
+    Each schedule's pass is timed --steps times from that state, then one
+    profiled pass gives its per-task cycles (makespan_of).  The smallest
+    bound over the schedules caps what any order or split of the same work
+    could gain over the plain schedule: plain pass / that bound."""
+    wname, seed, sd, k, desc = WORKLOADS[args.workload]
+    pairs = workload_pairs(args, gg, wname, seed, sd)
+    dev = kcl.device_info()
+    clock_hz, cus = dev["clock_khz"] * 1e3, dev["cu_count"]
+    g = kcl.Graph(pairs)
+    del pairs
+    log(f"bench makespan: {args.workload} n={g.n} m={g.m} core={g.core}; clock "
+        f"{clock_hz / 1e6:.0f} MHz, {cus} CUs")
+    model = kcl.CostModel(path=args.model or os.path.join(kcl.PKG, "cost_model.kclcm"))
+    main = kcl.Sim(g, k, schedule=kcl.SCHED_LEARNED, model=model)
+    t = 0
+    for _ in range(args.warmup):
+        t += 1
+        main.step(t)
+    rho = main.rho()
+    t += 1
+    out = {}
+    for name in ("degree", "counted", "learned"):
+        s = main if name == "learned" else kcl.Sim(g, k, schedule=kcl.SCHEDULES[name])
+        ms, counts = [], None
+        for _ in range(max(args.steps, 1)):
+            s.set_rho(rho)
+            s.step(t)
+            ms.append(s.last_pass()[2])
+            c = s.counts()
+            if counts is not None and not np.array_equal(c, counts):
+                raise RuntimeError(f"{name}: counts differ between repeated passes")
+            counts = c
+        if name == "degree":
+            ref_counts = counts
+        elif not np.array_equal(counts, ref_counts):
+            raise RuntimeError(f"{name}: counts differ from the plain schedule's")
+        s.set_rho(rho)
+        m = makespan_of(s, clock_hz, cus)
+        m["pass_ms"] = float(np.median(ms))
+        m["pass_over_lb"] = m["pass_ms"] / m["lb_ms"] if m["lb_ms"] > 0 else None
+        out[name] = m
+        log(f"bench makespan: {name}: pass {m['pass_ms']:.1f} ms (runs {ms}), bound "
+            f"{m['lb_ms']:.1f} ms, longest task {m['max_task_ms']:.1f} ms, {m['tasks']} tasks")
+        if s is not main:
+            s.close()
+    main.close()
+    model.close()
+    lb = min(v["lb_ms"] for v in out.values())
+    res = {"metric": "pass makespan vs lower bound, per schedule", "unit": "ms",
+           "workload": f"{args.workload}: {desc}", "k": k, "t": t, "clock_mhz": clock_hz / 1e6,
+           "cu_count": cus, "schedules": out, "lb_min_ms": lb,
+           "plain_over_lb_min": out["degree"]["pass_ms"] / lb if lb > 0 else None,
+           "learned_vs_plain": out["degree"]["pass_ms"] / out["learned"]["pass_ms"]}
+    g.close()
+    print(json.dumps(res), flush=True)
+
+
 def run_exact(args, kcl, gg):
     """Time-to-exact: text edge file -> graph -> certified maximal densest
     subgraph (kcl_exact_run), wall clock."""
@@ -462,7 +549,7 @@ def run_exact(args, kcl, gg):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=15)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="cl4m", choices=sorted(WORKLOADS) + ["batch512"])
     ap.add_argument("--no-tune", action="store_true",
@@ -475,9 +562,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--exact-budget", type=int, default=240,
                     help="exact mode: seconds before giving up on a certificate")
-    ap.add_argument("--mode", default="sim", choices=["sim", "exact", "trim"],
+    ap.add_argument("--mode", default="sim", choices=["sim", "exact", "trim", "makespan"],
                     help="sim: visits/s of one iteration (default); exact: time-to-exact; "
-                         "trim: k-clique core trimming time")
+                         "trim: k-clique core trimming time; makespan: each schedule's "
+                         "pass against a makespan lower bound")
     ap.add_argument("--gpu-peel-min", type=int, default=0,
                     help="trim mode: pops through at least this many cliques go to the GPU "
                          "(0: the library's default)")
@@ -511,6 +599,9 @@ def main():
         return
     if args.mode == "trim":
         run_trim(args, kcl, gg)
+        return
+    if args.mode == "makespan":
+        run_makespan(args, kcl, gg)
         return
     if args.workload == "batch512":
         run_batch(args, kcl, gg)
@@ -604,13 +695,18 @@ def roofline_of(workload: str, alg_bytes: int, kern_avg_ms: float) -> dict:
     measured (HBM traffic, VALU and LDS busy fractions, the bound)."""
     achieved = alg_bytes / (kern_avg_ms * 1e-3) / 1e9 if kern_avg_ms > 0 else 0.0
     traffic = profiled(workload, "traffic_bytes_per_pass")
-    out = {"bound": profiled(workload, "bound", text=True) or "hbm",
+    # The roof: HBM, the only one the integer pass has (it issues no MFMA).
+    # What actually binds is read from the committed counter profile of the
+    # workload ("limiter": valu / lds / latency), with the busy fractions.
+    out = {"bound": "hbm",
            "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS,
            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 6),
            "traffic": int(traffic) if traffic is not None else None,
            "kernel": "pass_kernel (all word classes of one enumeration pass)",
-           "kernel_ms": round(kern_avg_ms, 4), "algorithmic_bytes": alg_bytes}
-    for key in ("valu_busy_frac", "lds_busy_frac", "lds_bank_conflict_frac",
+           "kernel_ms": round(kern_avg_ms, 4), "algorithmic_bytes": alg_bytes,
+           "limiter": profiled(workload, "limiter", text=True),
+           "profile": f"profiles/pass_kernel_{workload}.md"}
+    for key in ("valu_busy_frac", "lds_busy_frac", "salu_busy_frac", "lds_bank_conflict_frac",
                 "hbm_traffic_frac"):
         v = profiled(workload, key)
         if v is not None:

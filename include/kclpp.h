@@ -214,6 +214,25 @@ kcl_status kcl_sim_root_cliques(kcl_sim *s, uint64_t *out);
 kcl_status kcl_sim_profile_pass(kcl_sim *s, uint32_t *ntasks, uint32_t *root, uint32_t *part,
                                 uint32_t *nparts, uint64_t *build_cycles, uint64_t *total_cycles);
 
+/* The launch shape of the plan's word classes (no counterpart in the
+ * reference, whose OpenMP loop has one kind of task).  Entries 0..8 are
+ * the vertex-task classes, 9..17 the edge-task classes (run by the (k-1)
+ * kernels); a class's tasks are the profile arrays' entries
+ * [first, first + count).  Used to price a pass's measured task cycles
+ * against the GPU's resident workgroups (a makespan lower bound). */
+#define KCL_NUM_CLASSES 9
+typedef struct kcl_class_info {
+  uint32_t words;     /* 64-bit words per bitmap row (members <= 64 words) */
+  uint32_t k;         /* clique size of the kernel (k, or k-1 for edge tasks) */
+  uint32_t first;     /* first task of the class in plan order */
+  uint32_t count;     /* tasks of the class */
+  uint32_t waves;     /* wavefronts per workgroup */
+  uint32_t per_cu;    /* resident workgroups per CU the launch assumes */
+  uint32_t blocks;    /* workgroups launched (min(CUs x per_cu, count)) */
+  uint32_t lds_bytes; /* LDS per workgroup */
+} kcl_class_info;
+kcl_status kcl_sim_plan_classes(kcl_sim *s, kcl_class_info out[2 * KCL_NUM_CLASSES]);
+
 /* ---------------------------------------------------------------------
  * Learned root-cost model (no counterpart in the reference, whose OpenMP
  * loop has no cost estimate).  A BF16 MLP 8 -> 64 -> 64 -> 2 (ReLU) run
@@ -504,6 +523,7 @@ typedef struct kcl_device_info {
   uint32_t cu_count;
   uint64_t hbm_bytes;
   uint32_t lds_bytes_per_cu;
+  uint32_t clock_khz; /* peak shader clock (the shader-clock timers' rate at most) */
 } kcl_device_info;
 
 kcl_status kcl_device_get_info(kcl_device_info *info);

@@ -35,6 +35,7 @@ CM_FEATURES = 8
 CM_OUTPUTS = 2
 CM_PARAMS = 4866
 CM_NORM = 20
+NUM_CLASSES = 9  # word classes of the enumeration kernels (KCL_NUM_CLASSES)
 
 _STATUS = {
     0: "KCL_OK", 1: "KCL_E_IO", 2: "KCL_E_PARSE", 3: "KCL_E_INVALID_ARG",
@@ -123,7 +124,13 @@ class _Record(C.Structure):
 
 class _DeviceInfo(C.Structure):
     _fields_ = [("name", C.c_char * 64), ("arch", C.c_char * 32), ("cu_count", C.c_uint32),
-                ("hbm_bytes", C.c_uint64), ("lds_bytes_per_cu", C.c_uint32)]
+                ("hbm_bytes", C.c_uint64), ("lds_bytes_per_cu", C.c_uint32),
+                ("clock_khz", C.c_uint32)]
+
+
+class _ClassInfo(C.Structure):
+    _fields_ = [(f, C.c_uint32) for f in ("words", "k", "first", "count", "waves", "per_cu",
+                                          "blocks", "lds_bytes")]
 
 
 _lib = None
@@ -136,7 +143,8 @@ EXPORTS = [
     "kcl_graph_csr", "kcl_graph_edges", "kcl_count_edges", "kcl_clique_enum",
     "kcl_clique_enum_noalloc", "kcl_sim_create",
     "kcl_sim_destroy", "kcl_sim_step", "kcl_sim_counts", "kcl_sim_rho", "kcl_sim_set_rho",
-    "kcl_sim_totals", "kcl_sim_last_pass", "kcl_sim_plan_info", "kcl_sim_root_cliques", "kcl_sim_profile_pass", "kcl_sim_find_densest", "kcl_cmpsync_run",
+    "kcl_sim_totals", "kcl_sim_last_pass", "kcl_sim_plan_info", "kcl_sim_root_cliques", "kcl_sim_profile_pass",
+    "kcl_sim_plan_classes", "kcl_sim_find_densest", "kcl_cmpsync_run",
     "kcl_device_get_info", "kcl_version", "kcl_trim_create", "kcl_trim_create_opts",
     "kcl_trim_destroy",
     "kcl_trim_get_info", "kcl_trim_nodes", "kcl_trim_edges", "kcl_trim_write", "kcl_exact_run",
@@ -257,7 +265,8 @@ def device_info() -> dict:
     d = _DeviceInfo()
     _check(lib().kcl_device_get_info(C.byref(d)))
     return {"name": d.name.decode(), "arch": d.arch.decode(), "cu_count": d.cu_count,
-            "hbm_bytes": d.hbm_bytes, "lds_bytes_per_cu": d.lds_bytes_per_cu}
+            "hbm_bytes": d.hbm_bytes, "lds_bytes_per_cu": d.lds_bytes_per_cu,
+            "clock_khz": d.clock_khz}
 
 
 def read_edge_list(path: str) -> tuple[np.ndarray, int]:
@@ -578,6 +587,19 @@ class Sim:
                                           _ptr(out["part"]), _ptr(out["nparts"]),
                                           _ptr(out["build_cycles"]), _ptr(out["total_cycles"])))
         return {k: v[:n] for k, v in out.items()}
+
+    def plan_classes(self) -> list[dict]:
+        """Launch shape of the plan's word classes: vertex-task classes,
+        then edge-task classes (only those with tasks)."""
+        arr = (_ClassInfo * (2 * NUM_CLASSES))()
+        _check(lib().kcl_sim_plan_classes(self._h, arr))
+        out = []
+        for i, c in enumerate(arr):
+            if c.count:
+                d = {f: int(getattr(c, f)) for f, _ in _ClassInfo._fields_}
+                d["edge"] = i >= NUM_CLASSES
+                out.append(d)
+        return out
 
     def root_features(self) -> np.ndarray:
         """Cost-model features of every node as a root ((n, 8) float32)."""

@@ -216,6 +216,12 @@ void promote_plan(PassPlan *plan, uint32_t min_class);
 // Concurrent task slots of a word class's k kernel in one launch
 // (workgroups).
 uint32_t class_slots(uint32_t wclass, uint32_t k);
+// Launch shape of a class kernel in a pass (as launch_pass_raw picks it):
+// waves per workgroup, workgroups per CU, workgroups launched for `count`
+// tasks, LDS bytes per workgroup.
+kcl_status class_launch_shape(uint32_t wclass, uint32_t k, bool all, uint32_t max_per_cu,
+                              uint32_t count, uint32_t *waves, uint32_t *per_cu,
+                              uint32_t *blocks, uint32_t *lds);
 
 // Plan for (graph, k): root filter (a root-credit k-pass), costs by the
 // schedule, split tasks.  root_cliques (nullable) receives each root's

@@ -12,13 +12,14 @@ namespace {
 struct PassTable {
   PassFn fn[kNumClasses][kMaxTemplK + 1] = {};
   PassFn fn_all[kNumClasses][kMaxTemplK + 1] = {};
+  PassFn deep[kNumClasses][2] = {};  // large k: credit winner/root, credit all
   size_t lds[kNumClasses] = {};
   PassTable() {
-    void (*fill[kNumClasses])(PassFn *, PassFn *, size_t *) = {
+    void (*fill[kNumClasses])(PassFn *, PassFn *, PassFn *, size_t *) = {
         fill_pass_class_0, fill_pass_class_1, fill_pass_class_2, fill_pass_class_3,
         fill_pass_class_4, fill_pass_class_5, fill_pass_class_6, fill_pass_class_7,
         fill_pass_class_8};
-    for (uint32_t c = 0; c < kNumClasses; ++c) fill[c](fn[c], fn_all[c], &lds[c]);
+    for (uint32_t c = 0; c < kNumClasses; ++c) fill[c](fn[c], fn_all[c], deep[c], &lds[c]);
   }
 };
 
@@ -28,15 +29,37 @@ const PassTable &pass_table() {
 }
 
 PassFn kernel_for(uint32_t wclass, uint32_t k, bool all) {
-  if (wclass >= kNumClasses || k < 2 || k > (uint32_t)kMaxTemplK) return nullptr;
+  if (wclass >= kNumClasses || k < 2 || k > kMaxDeepK) return nullptr;
+  if (k > (uint32_t)kMaxTemplK) return pass_table().deep[wclass][all ? 1 : 0];
   return all ? pass_table().fn_all[wclass][k] : pass_table().fn[wclass][k];
+}
+
+// Waves per workgroup of the class-c kernel for k.
+uint32_t waves_for(uint32_t wclass, uint32_t k, bool all) {
+  return k > (uint32_t)kMaxTemplK ? deep_waves(wclass) : class_waves(wclass, k, all);
+}
+
+// Large k: each wavefront's path (and, for k-clique degrees, per-level
+// counts) must fit its share of the class's key array.
+bool deep_fits(uint32_t wclass, uint32_t k, bool all) {
+  if (k <= (uint32_t)kMaxTemplK) return true;
+  const uint32_t W = kClassWords[wclass];
+  return deep_words(k - 3, all) <= 64u * W / deep_waves(wclass);
+}
+
+// The class whose kernel runs class c's tasks: c itself, or for large k the
+// narrowest wider class with room for the walk (a task fits any wider
+// class); kNumClasses if none has.
+uint32_t launch_class(uint32_t c, uint32_t k, bool all) {
+  while (c < kNumClasses && !deep_fits(c, k, all)) ++c;
+  return c;
 }
 
 size_t lds_bytes(uint32_t wclass) { return pass_table().lds[wclass]; }
 
 }  // namespace
 
-uint32_t max_pass_k() { return (uint32_t)kMaxTemplK; }
+uint32_t max_pass_k() { return kMaxDeepK; }
 
 namespace {
 
@@ -63,10 +86,31 @@ __global__ void zero_tickets_kernel(uint32_t *t) {
 
 }  // namespace
 
+kcl_status class_launch_shape(uint32_t wclass, uint32_t k, bool all, uint32_t max_per_cu,
+                              uint32_t count, uint32_t *waves, uint32_t *per_cu,
+                              uint32_t *blocks, uint32_t *lds) {
+  DeviceProps props;
+  KCL_TRY(device_props(&props));
+  if (wclass >= kNumClasses) return fail(KCL_E_INVALID_ARG, "word class %u", wclass);
+  const uint32_t lc = launch_class(wclass, k, all);
+  if (lc >= kNumClasses) return fail(KCL_E_UNSUPPORTED, "k = %u: no word class holds the walk", k);
+  wclass = lc;
+  const uint32_t w = waves_for(wclass, k, all);
+  uint32_t pc = blocks_per_cu(wclass, props, w);
+  if (max_per_cu && max_per_cu < pc) pc = max_per_cu;
+  uint64_t b = (uint64_t)props.cu_count * pc;
+  if (b > count) b = count;
+  *waves = w;
+  *per_cu = pc;
+  *blocks = (uint32_t)b;
+  *lds = (uint32_t)lds_bytes(wclass);
+  return KCL_OK;
+}
+
 uint32_t class_slots(uint32_t wclass, uint32_t k) {
   DeviceProps props;
   if (wclass >= kNumClasses || device_props(&props) != KCL_OK) return 1;
-  return props.cu_count * blocks_per_cu(wclass, props, class_waves(wclass, k));
+  return props.cu_count * blocks_per_cu(wclass, props, waves_for(wclass, k, false));
 }
 
 kcl_status launch_pass(const kcl_graph *g, const uint64_t *d_key, uint32_t k, PassMode mode,
@@ -109,10 +153,18 @@ kcl_status launch_pass_raw(const kcl_graph *g, const uint64_t *d_key, uint32_t k
     for (int c = (int)kNumClasses - 1; c >= 0; --c) {  // heaviest word class first
       const uint32_t count = cnts[c];
       if (!count) continue;
-      PassFn fn = kernel_for((uint32_t)c, kk, mode == kCreditAll);
+      const bool all = mode == kCreditAll;
+      const uint32_t lc = launch_class((uint32_t)c, kk, all);
+      if (lc >= kNumClasses)
+        return fail(KCL_E_UNSUPPORTED, "k = %u: no word class has room for a %u-level %s walk",
+                    kk, kk - 3, all ? "k-clique-degree" : "counting");
+      PassFn fn = kernel_for(lc, kk, all);
       if (!fn) return fail(KCL_E_UNSUPPORTED, "no enumeration kernel for k = %u", kk);
-      const uint32_t waves = class_waves((uint32_t)c, kk, mode == kCreditAll);
-      uint32_t per_cu = blocks_per_cu((uint32_t)c, props, waves);
+      const uint32_t waves = waves_for(lc, kk, all);
+      // Large-k kernels read the clique size from the bits above the mode.
+      const uint32_t mode_arg =
+          kk > (uint32_t)kMaxTemplK ? ((uint32_t)mode | (kk << 8)) : (uint32_t)mode;
+      uint32_t per_cu = blocks_per_cu(lc, props, waves);
       if (plan.max_per_cu && plan.max_per_cu < per_cu) per_cu = plan.max_per_cu;
       uint64_t blocks = (uint64_t)props.cu_count * per_cu;
       if (blocks > count) blocks = count;
@@ -124,7 +176,7 @@ kcl_status launch_pass_raw(const kcl_graph *g, const uint64_t *d_key, uint32_t k
                          g->core_h, plan.tasks + begin[c], count,
                          grp ? count : std::min(plan.light_from[c], count), g->dense_items,
                          plan.tickets + 2 * (grp * kNumClasses + c), plan.tickets + kTickets,
-                         plan.list, plan.nlist, (uint32_t)mode, d_cnt,
+                         plan.list, plan.nlist, mode_arg, d_cnt,
                          prof ? prof + 2 * (size_t)begin[c] : nullptr);
       KCL_HIP(hipGetLastError());
     }

@@ -25,10 +25,13 @@ struct KTable {
 #define KCL_CAT2(a, b) a##b
 #define KCL_CAT(a, b) KCL_CAT2(a, b)
 
-void KCL_CAT(fill_pass_class_, KCL_PASS_CLASS)(PassFn *fn, PassFn *fn_all, size_t *lds) {
+void KCL_CAT(fill_pass_class_, KCL_PASS_CLASS)(PassFn *fn, PassFn *fn_all, PassFn *deep,
+                                               size_t *lds) {
   constexpr int C = KCL_PASS_CLASS;
   constexpr int W = (int)kClassWords[C];
   KTable<C, 2>::fill(fn, fn_all);
+  deep[0] = &pass_kernel<W, 0, (int)deep_waves(C), false>;
+  deep[1] = &pass_kernel<W, 0, (int)deep_waves(C), true>;
   *lds = sizeof(Lds<W>);
 }
 

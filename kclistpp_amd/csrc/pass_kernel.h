@@ -48,6 +48,18 @@
 // next set is formed from that word on (row words moved to scalar
 // registers, one scalar AND each) and its edges counted from there.  Rows
 // of 8 or more words (roots inside a dense core) are always counted so.
+//
+// Large k (K = 0 instantiation: the clique size is a launch argument, for
+// k above the compiled ones).  Ranked rows as above; a wavefront per item u
+// walks the cliques of A'[u] depth first with the current set spread over
+// the lanes (lane j holds word j), choosing each level's next member with a
+// ballot.  Only the chosen members are kept (a u16 path per level in the
+// LDS the keys used before ranking); a level's remaining set is recomputed
+// from the path when the walk returns to it.  The last two levels are the
+// edges of the final set, shared by the lanes as in the dense counting.
+// Values one lane holds reach the others through readlane, never through
+// LDS: the compiler orders a lane's LDS accesses only against its own.
+// Every lane writes the same path entry, so each reads back its own write.
 #pragma once
 
 #include "kcl_internal.h"
@@ -303,6 +315,34 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
   return v;
 }
 
+__device__ __forceinline__ uint32_t wave_sum32(uint32_t v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+// Lane `src`'s 64-bit value in every lane (src wave-uniform).
+__device__ __forceinline__ unsigned long long read_lane64(unsigned long long x, uint32_t src) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)x, (int)src);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(x >> 32), (int)src);
+  return ((unsigned long long)hi << 32) | lo;
+}
+
+// Word j of the set of members ranked after p.
+__device__ __forceinline__ unsigned long long after_mask(uint32_t p, uint32_t j) {
+  const uint32_t pw = p >> 6, pb = p & 63;
+  if (j > pw) return ~0ull;
+  if (j < pw || pb == 63) return 0ull;
+  return ~0ull << (pb + 1);
+}
+
+// Large-k layout of a wavefront's share of the key array (used once the
+// members are ranked), in u64 words: (k-clique degrees only) one count per
+// level, then one u16 member per level.
+__host__ __device__ constexpr uint32_t deep_words(uint32_t levels, bool all) {
+  return (all ? levels : 0u) + (levels + 3u) / 4u;
+}
+
 // In-place inclusive scan of a[0..d) by one wavefront.
 __device__ __forceinline__ void wave_scan(uint32_t *a, uint32_t d, uint32_t lane) {
   const uint32_t chunk = (d + 63) >> 6;
@@ -340,8 +380,14 @@ __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restric
                                                        uint32_t mode,
                                                        unsigned long long *__restrict__ cnt,
                                                        unsigned long long *__restrict__ prof) {
-  static_assert(K >= 2, "k >= 2");
+  static_assert(K == 0 || K >= 2, "k >= 2, or 0: k from the launch (large k)");
   static_assert(NW >= 1 && NW <= 16, "at most 16 wavefronts per workgroup");
+  // Large-k kernels take the clique size in the bits above the mode.
+  uint32_t kdeep = 0;
+  if constexpr (K == 0) {
+    kdeep = mode >> 8;
+    mode &= 0xffu;
+  }
   constexpr uint32_t NT = 64 * NW;  // threads of the workgroup sharing one root
   constexpr uint32_t RS = (uint32_t)row_stride<W>();
   __shared__ Lds<W> sh;
@@ -352,8 +398,8 @@ __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restric
   // (key, id) rank before building the rows; their dense roots are counted
   // a wavefront per item.  Narrower rows (d <= 192) hardly ever reach the
   // dense threshold, so they skip the ranking.
-  constexpr bool kRanked = (K >= 6 && W >= 4);
-  constexpr bool kDense = kRanked && !ALL;
+  constexpr bool kRanked = (K >= 6 && W >= 4) || K == 0;
+  constexpr bool kDense = kRanked && !ALL && K != 0;
   // Rows of 8+ words belong to roots inside a dense core (d+ > 448): those
   // kernels count every task densely and leave out the per-lane path, whose
   // registers would halve their occupancy.
@@ -486,7 +532,7 @@ __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restric
       sh.root_sum = 0;
       sh.nwin = 0;
     }
-    if constexpr (K >= 3) {
+    if constexpr (K >= 3 || K == 0) {
       for (uint32_t i = tid; i < d * RS; i += NT) sh.rows[i] = 0;
     }
     __syncthreads();
@@ -632,7 +678,126 @@ __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restric
       __syncthreads();
       if (prof && tid == 0) prof[2 * (size_t)ti] = __builtin_amdgcn_s_memtime() - t_start;
 
-      if constexpr (K == 3) {
+      if constexpr (K == 0) {
+        // Large k: a wavefront per item u (ranks of this part), the
+        // (kdeep-2)-cliques of A'[u] walked depth first.  c = members still
+        // to choose inside the current set; at c = 3 the chosen member's set
+        // is final and its edges are counted by the lanes.
+        const uint32_t lo = (uint32_t)((uint64_t)part * d / np);
+        const uint32_t hi = (uint32_t)((uint64_t)(part + 1) * d / np);
+        if (tid == 0) sh.next_item = lo;
+        __syncthreads();
+        const uint32_t wid = tid >> 6;
+        const uint32_t levels = kdeep - 3;  // path / count slots, 1-based
+        unsigned long long *ws =
+            reinterpret_cast<unsigned long long *>(sh.key) + wid * (Lds<W>::D / NW);
+        unsigned long long *lacc = ws;  // [levels] (k-clique degrees; lane 0 only)
+        uint16_t *path = reinterpret_cast<uint16_t *>(ws + (ALL ? levels : 0u));  // [levels]
+        const bool word_lane = lane < (uint32_t)W;
+        const uint32_t nwin = sh.nwin;
+        for (;;) {
+          uint32_t it = 0;
+          if (lane == 0) it = atomicAdd(&sh.next_item, 1u);
+          it = (uint32_t)__builtin_amdgcn_readfirstlane((int)it);
+          if (it >= hi) break;
+          const uint32_t u = it;
+          unsigned long long R = word_lane ? sh.rows[u * RS + lane] : 0ull;
+          uint32_t l = 1, c = kdeep - 2;
+          unsigned long long item_total = 0;
+          if constexpr (ALL) {
+            if (lane == 0) lacc[1] = 0;
+          }
+          for (;;) {
+            const unsigned long long nz = __ballot(R != 0ull);
+            if (nz == 0ull) {
+              if (l == 1) break;
+              // Level l is done: back to level l-1, whose remaining set is
+              // A'[u] & A'[path[1]] & ... & A'[path[l-2]], after path[l-1].
+              --l;
+              ++c;
+              if constexpr (ALL) {
+                if (lane == 0) {
+                  const unsigned long long v = lacc[l + 1];
+                  if (v) {
+                    atomicAdd(&sh.cred[path[l]], v);
+                    lacc[l] += v;
+                  }
+                }
+              }
+              unsigned long long S = word_lane ? sh.rows[u * RS + lane] : 0ull;
+              for (uint32_t i = 1; i < l; ++i)
+                S &= word_lane ? sh.rows[(uint32_t)path[i] * RS + lane] : 0ull;
+              R = S & after_mask(path[l], lane);
+              continue;
+            }
+            const uint32_t j0 = (uint32_t)__builtin_ctzll(nz);
+            const unsigned long long word = read_lane64(R, j0);
+            const uint32_t w = 64 * j0 + (uint32_t)__builtin_ctzll(word);
+            if (lane == j0) R &= R - 1ull;
+            const unsigned long long N = word_lane ? (R & sh.rows[w * RS + lane]) : 0ull;
+            const uint32_t cn = c - 1;  // members still to choose inside N
+            if (wave_sum32((uint32_t)__popcll(N)) < cn) continue;
+            if (cn == 2) {
+              // The edges of N, its words moved to scalar registers: lane l
+              // takes member 64 j + l of each word j.
+              unsigned long long F[W];
+#pragma unroll
+              for (int j = 0; j < W; ++j) F[j] = read_lane64(N, (uint32_t)j);
+              uint32_t ce = 0;
+              if constexpr (ALL) {
+#pragma unroll
+                for (int j = 0; j < W; ++j) {
+                  if (F[j] == 0ull || !((F[j] >> lane) & 1ull)) continue;
+                  const unsigned long long *ry = sh.rows + (64 * (uint32_t)j + lane) * RS;
+                  uint32_t cy = 0;
+#pragma unroll
+                  for (int jj = j; jj < W; ++jj) {
+                    unsigned long long e = F[jj] & ry[jj];
+                    cy += (uint32_t)__popcll(e);
+                    while (e) {
+                      atomicAdd(&sh.cred[64 * (uint32_t)jj + (uint32_t)__builtin_ctzll(e)], 1ull);
+                      e &= e - 1ull;
+                    }
+                  }
+                  if (cy) atomicAdd(&sh.cred[64 * (uint32_t)j + lane], (unsigned long long)cy);
+                  ce += cy;
+                }
+              } else {
+                ce = wave_edges_from<W, 0>(F, sh.rows, lane);
+              }
+              const unsigned long long cb = wave_sum((unsigned long long)ce);
+              if constexpr (ALL) {
+                if (lane == 0 && cb) {
+                  atomicAdd(&sh.cred[w], cb);
+                  lacc[l] += cb;
+                }
+              } else {
+                item_total += cb;
+              }
+              continue;
+            }
+            path[l] = (uint16_t)w;  // the same value from every lane
+            ++l;
+            c = cn;
+            if constexpr (ALL) {
+              if (lane == 0) lacc[l] = 0;
+            }
+            R = N;
+          }
+          if (lane == 0) {
+            if constexpr (ALL) {
+              const unsigned long long v = lacc[1];
+              if (v) {
+                atomicAdd(&sh.cred[u], v);
+                root_acc += v;
+              }
+            } else if (item_total) {
+              if (!to_root && u < nwin) atomicAdd(&sh.cred[u], item_total);
+              else root_acc += item_total;
+            }
+          }
+        }
+      } else if constexpr (K == 3) {
         const uint32_t lo = (uint32_t)((uint64_t)part * d / np);
         const uint32_t hi = (uint32_t)((uint64_t)(part + 1) * d / np);
         for (uint32_t u = lo + tid; u < hi; u += NT) {
@@ -898,18 +1063,28 @@ using PassFn = void (*)(const uint32_t *, const uint32_t *, const uint64_t *,
                         unsigned long long *, unsigned long long *);
 
 constexpr int kMaxTemplK = 10;  // k handled by compiled kernels: 2..kMaxTemplK
+constexpr uint32_t kMaxDeepK = 255;  // larger k (the reference's unsigned char k)
+
+// Waves per workgroup of the large-k kernels: one for rows of up to 128
+// members, two up to 256, four wider (each wavefront keeps its path in its
+// share of the key array).
+constexpr uint32_t deep_waves(uint32_t c) {
+  return kClassWords[c] <= 2 ? 1u : kClassWords[c] <= 4 ? 2u : 4u;
+}
 
 // Kernel pointers and LDS size of word class <c> (one translation unit per
-// class, pass_inst.hip, so the instantiations compile in parallel).
-void fill_pass_class_0(PassFn *, PassFn *, size_t *);
-void fill_pass_class_1(PassFn *, PassFn *, size_t *);
-void fill_pass_class_2(PassFn *, PassFn *, size_t *);
-void fill_pass_class_3(PassFn *, PassFn *, size_t *);
-void fill_pass_class_4(PassFn *, PassFn *, size_t *);
-void fill_pass_class_5(PassFn *, PassFn *, size_t *);
-void fill_pass_class_6(PassFn *, PassFn *, size_t *);
-void fill_pass_class_7(PassFn *, PassFn *, size_t *);
-void fill_pass_class_8(PassFn *, PassFn *, size_t *);
+// class, pass_inst.hip, so the instantiations compile in parallel): the
+// k = 2..kMaxTemplK kernels, their k-clique-degree versions, and the
+// large-k kernel and its degree version.
+void fill_pass_class_0(PassFn *, PassFn *, PassFn *, size_t *);
+void fill_pass_class_1(PassFn *, PassFn *, PassFn *, size_t *);
+void fill_pass_class_2(PassFn *, PassFn *, PassFn *, size_t *);
+void fill_pass_class_3(PassFn *, PassFn *, PassFn *, size_t *);
+void fill_pass_class_4(PassFn *, PassFn *, PassFn *, size_t *);
+void fill_pass_class_5(PassFn *, PassFn *, PassFn *, size_t *);
+void fill_pass_class_6(PassFn *, PassFn *, PassFn *, size_t *);
+void fill_pass_class_7(PassFn *, PassFn *, PassFn *, size_t *);
+void fill_pass_class_8(PassFn *, PassFn *, PassFn *, size_t *);
 static_assert(kNumClasses == 9, "one fill_pass_class_<c> per word class");
 
 }  // namespace kcl

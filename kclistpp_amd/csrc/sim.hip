@@ -460,6 +460,27 @@ extern "C" kcl_status kcl_sim_last_pass(kcl_sim *s, uint64_t *visits, double *ke
   return KCL_OK;
 }
 
+extern "C" kcl_status kcl_sim_plan_classes(kcl_sim *s, kcl_class_info *out) {
+  if (!s || !out) return fail(KCL_E_INVALID_ARG, "NULL argument");
+  static_assert(KCL_NUM_CLASSES == kcl::kNumClasses, "class count of the ABI");
+  for (uint32_t grp = 0; grp < 2; ++grp) {
+    const uint32_t *begin = grp ? s->plan.eclass_begin : s->plan.class_begin;
+    const uint32_t *count = grp ? s->plan.eclass_count : s->plan.class_count;
+    const uint32_t kk = grp ? s->k - 1 : s->k;
+    for (uint32_t c = 0; c < kcl::kNumClasses; ++c) {
+      kcl_class_info &o = out[grp * kcl::kNumClasses + c];
+      o = kcl_class_info{};
+      o.words = kcl::kClassWords[c];
+      o.k = kk;
+      o.first = begin[c];
+      o.count = s->trivial ? 0u : count[c];
+      KCL_TRY(kcl::class_launch_shape(c, kk, false, s->plan.max_per_cu, o.count, &o.waves,
+                                      &o.per_cu, &o.blocks, &o.lds_bytes));
+    }
+  }
+  return KCL_OK;
+}
+
 extern "C" kcl_status kcl_sim_profile_pass(kcl_sim *s, uint32_t *ntasks, uint32_t *root,
                                            uint32_t *part, uint32_t *nparts, uint64_t *build_cycles,
                                            uint64_t *total_cycles) {
@@ -694,5 +715,6 @@ extern "C" kcl_status kcl_device_get_info(kcl_device_info *info) {
   info->cu_count = (uint32_t)p.multiProcessorCount;
   info->hbm_bytes = (uint64_t)p.totalGlobalMem;
   info->lds_bytes_per_cu = (uint32_t)p.maxSharedMemoryPerMultiProcessor;
+  info->clock_khz = (uint32_t)p.clockRate;
   return KCL_OK;
 }
