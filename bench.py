@@ -497,6 +497,71 @@ def run_makespan(args, kcl, gg):
     print(json.dumps(res), flush=True)
 
 
+def run_sampling(args, kcl, gg):
+    """Sampling++ (kCDensestSamplingKclistpp): count, keep ~1e7 cliques,
+    T = --steps rounds with reports after 1, 2, 4, ...; wall clock from the
+    edges in memory to the last report.  cpu_baseline: the reference program
+    (built from its sources, all host threads) on the same text file, with
+    a time limit."""
+    import subprocess
+    wname, seed, sd, k, desc = WORKLOADS[args.workload]
+    pairs = workload_pairs(args, gg, wname, seed, sd)
+    T = args.steps
+    log(f"bench sampling: {args.workload} ({desc}), {pairs.shape[0]} edges, T = {T}")
+    t0 = time.perf_counter()
+    with kcl.Graph(pairs) as g:
+        t1 = time.perf_counter()
+        with kcl.Sampling(g, k, seed=1) as smp:
+            t2 = time.perf_counter()
+            recs = smp.run(T)
+            info = smp.info
+        t3 = time.perf_counter()
+    log(f"bench sampling: graph {t1 - t0:.2f}s, count+sample {t2 - t1:.2f}s, rounds+reports "
+        f"{t3 - t2:.2f}s; {info}; last {recs[-1]}")
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import REF_SAMPLING_BIN
+        with tempfile.TemporaryDirectory() as td:
+            path = os.path.join(td, "edges.txt")
+            gg.write_text(pairs, path)
+            c0 = time.perf_counter()
+            # The reference's own lines are passed on as it prints them.
+            proc = subprocess.Popen([REF_SAMPLING_BIN, str(host_threads()), str(T), str(k), path],
+                                    cwd=td, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                                    text=True, bufsize=1)
+            lines, ok = [], False
+            try:
+                for line in proc.stdout:
+                    log("reference: " + line.rstrip())
+                    if line.startswith("Approximate"):
+                        lines.append(line.rstrip())
+                    if time.perf_counter() - c0 > args.cpu_timeout:
+                        break
+                ok = proc.wait(timeout=max(1.0, args.cpu_timeout - (time.perf_counter() - c0))) == 0
+            except subprocess.TimeoutExpired:
+                ok = False
+            if proc.poll() is None:
+                proc.kill()
+                proc.wait()
+                ok = False
+            cs = time.perf_counter() - c0
+        cpu = {"value": cs, "unit": "s", "cores": host_threads(), "kind": "reference",
+               "cpu_model": cpu_model(),
+               "sample": f"the reference kCDensestSamplingKclistpp ({host_threads()} OpenMP "
+                         f"threads) on the same edge file, T = {T}: "
+                         f"{'finished' if ok else 'stopped at the time limit'} after {cs:.1f}s"
+                         + ("" if ok else " (a lower bound)"),
+               "last_line": lines[-1] if lines else None}
+        log(f"bench sampling: reference {cs:.1f}s ({'finished' if ok else 'time limit'})")
+    out = {"metric": f"Sampling++ time (k={k}, T={T})", "value": t3 - t0, "unit": "s",
+           "n_gpus": 1, "higher_is_better": False, "vs_baseline": None, "dtype": "u64",
+           "data": "synthetic (seeded generator, kclistpp_amd/graphgen.py)",
+           "config": {"workload": f"{args.workload}: {desc}", "k": k, "T": T},
+           "phases_s": {"graph": t1 - t0, "count_sample": t2 - t1, "rounds_reports": t3 - t2},
+           "result": {"info": info, "records": recs}, "cpu_baseline": cpu}
+    print(json.dumps(out), flush=True)
+
+
 def run_exact(args, kcl, gg):
     """Time-to-exact: text edge file -> graph -> certified maximal densest
     subgraph (kcl_exact_run), wall clock."""
@@ -562,10 +627,12 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--exact-budget", type=int, default=240,
                     help="exact mode: seconds before giving up on a certificate")
-    ap.add_argument("--mode", default="sim", choices=["sim", "exact", "trim", "makespan"],
+    ap.add_argument("--mode", default="sim",
+                    choices=["sim", "exact", "trim", "makespan", "sampling"],
                     help="sim: visits/s of one iteration (default); exact: time-to-exact; "
                          "trim: k-clique core trimming time; makespan: each schedule's "
-                         "pass against a makespan lower bound")
+                         "pass against a makespan lower bound; sampling: Sampling++ time "
+                         "(--steps rounds)")
     ap.add_argument("--gpu-peel-min", type=int, default=0,
                     help="trim mode: pops through at least this many cliques go to the GPU "
                          "(0: the library's default)")
@@ -602,6 +669,9 @@ def main():
         return
     if args.mode == "makespan":
         run_makespan(args, kcl, gg)
+        return
+    if args.mode == "sampling":
+        run_sampling(args, kcl, gg)
         return
     if args.workload == "batch512":
         run_batch(args, kcl, gg)

@@ -515,6 +515,67 @@ kcl_status kcl_exact_run(kcl_graph *g, uint32_t k, const kcl_exact_opts *opts,
                          uint32_t *nodes_out);
 
 /* ---------------------------------------------------------------------
+ * Sampling++ (kCDensestSamplingKclistpp.c, "Seq-Sampling++"): replaces
+ * SampleCliques + Solve (:438-489).  The count is the enumeration's; each
+ * k-clique is kept with probability p = min(1, target / count) by a seeded
+ * hash of its ids (the reference uses rand() seeded with the clock, :492),
+ * so a (graph, k, target, seed) gives one kept set on any schedule.  The
+ * "++" rounds are synchronous: in a round every kept clique gives one unit
+ * to its member of smallest load after the previous round, ties to the
+ * smaller id (the reference runs them one clique at a time in a shuffled
+ * order, :345-359).  Reports follow ExtractDensest and the line printed at
+ * :475-479: the densest prefix by decreasing load, its induced subgraph
+ * rebuilt and its k-cliques counted exactly, the scaled upper bound.
+ * ------------------------------------------------------------------- */
+typedef struct kcl_sampling kcl_sampling;
+
+typedef struct kcl_sampling_opts {
+  uint64_t target; /* cliques to keep on average; 0 = 1e7 (CLIQUES_TO_SAMPLE, :138) */
+  uint64_t seed;   /* the sampler's seed */
+  uint32_t reserved[4];
+} kcl_sampling_opts;
+
+typedef struct kcl_sampling_info {
+  uint64_t total;   /* k-cliques of the graph */
+  uint64_t sampled; /* cliques kept */
+  double prob;      /* keep probability */
+  uint32_t rounds;  /* rounds run */
+} kcl_sampling_info;
+
+typedef struct kcl_sampling_record {
+  uint32_t T;              /* rounds so far */
+  uint32_t n;              /* nodes of the approximate densest subgraph (after Relabel) */
+  uint32_t m;              /* its edges */
+  uint32_t prefix;         /* densest prefix of the load order (ExtractDensest's n) */
+  uint64_t prefix_sampled; /* kept cliques credited to the prefix (ExtractDensest's m) */
+  uint64_t cliques;        /* k-cliques of the subgraph, exact */
+  double edge_density;     /* 2 m / n / (n - 1) */
+  double density;          /* cliques / n */
+  double ub_sample;        /* ExtractDensest's upper bound over the sample */
+  double ub;               /* ub_sample / p / (1 - sqrt(6 ln(graph n) / ub_sample)) */
+  double seconds;          /* kcl_sampling_run: since its start */
+} kcl_sampling_record;
+
+/* Counts, then lists the kept cliques into device memory. */
+kcl_status kcl_sampling_create(kcl_graph *g, uint32_t k, const kcl_sampling_opts *opts,
+                               kcl_sampling **out);
+void kcl_sampling_destroy(kcl_sampling *s);
+kcl_status kcl_sampling_get_info(kcl_sampling *s, kcl_sampling_info *info);
+/* The kept cliques (sampled x k ids, each clique ascending; order unspecified). */
+kcl_status kcl_sampling_cliques(kcl_sampling *s, uint32_t *out);
+/* One synchronous "++" round. */
+kcl_status kcl_sampling_step(kcl_sampling *s);
+/* Loads per node (n entries): units received so far. */
+kcl_status kcl_sampling_loads(kcl_sampling *s, uint64_t *out);
+/* The report after the rounds run so far; order (nullable, n entries): the
+ * nodes by decreasing load. */
+kcl_status kcl_sampling_report(kcl_sampling *s, kcl_sampling_record *rec, uint32_t *order);
+/* Solve (:446-489): rounds up to num_iter with a report after rounds 1, 2,
+ * 4, ...; *nrec: in = room in recs, out = reports made. */
+kcl_status kcl_sampling_run(kcl_sampling *s, uint32_t num_iter, kcl_sampling_record *recs,
+                            uint32_t *nrec);
+
+/* ---------------------------------------------------------------------
  * Library information.
  * ------------------------------------------------------------------- */
 typedef struct kcl_device_info {

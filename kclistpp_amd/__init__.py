@@ -116,6 +116,22 @@ class _ExactResult(C.Structure):
                 ("degree_passes", C.c_uint32), ("host_seconds", C.c_double)]
 
 
+class _SamplingOpts(C.Structure):
+    _fields_ = [("target", C.c_uint64), ("seed", C.c_uint64), ("reserved", C.c_uint32 * 4)]
+
+
+class _SamplingInfo(C.Structure):
+    _fields_ = [("total", C.c_uint64), ("sampled", C.c_uint64), ("prob", C.c_double),
+                ("rounds", C.c_uint32)]
+
+
+class _SamplingRecord(C.Structure):
+    _fields_ = [("T", C.c_uint32), ("n", C.c_uint32), ("m", C.c_uint32), ("prefix", C.c_uint32),
+                ("prefix_sampled", C.c_uint64), ("cliques", C.c_uint64),
+                ("edge_density", C.c_double), ("density", C.c_double), ("ub_sample", C.c_double),
+                ("ub", C.c_double), ("seconds", C.c_double)]
+
+
 class _Record(C.Structure):
     _fields_ = [("T", C.c_uint32), ("n", C.c_uint32), ("m", C.c_uint32),
                 ("density", C.c_double), ("ub", C.c_double), ("sum", C.c_uint64),
@@ -155,7 +171,9 @@ EXPORTS = [
     "kcl_batch_get_info", "kcl_batch_graph_cliques", "kcl_batch_step", "kcl_batch_reset",
     "kcl_batch_last_pass",
     "kcl_batch_counts", "kcl_batch_rho", "kcl_batch_totals", "kcl_batch_find_densest",
-    "kcl_batch_run",
+    "kcl_batch_run", "kcl_sampling_create", "kcl_sampling_destroy", "kcl_sampling_get_info",
+    "kcl_sampling_cliques", "kcl_sampling_step", "kcl_sampling_loads", "kcl_sampling_report",
+    "kcl_sampling_run",
 ]
 
 
@@ -238,6 +256,15 @@ def lib() -> C.CDLL:
         "kcl_batch_totals": ([vp, u32, vp], C.c_int),
         "kcl_batch_find_densest": ([vp, C.POINTER(_Densest)], C.c_int),
         "kcl_batch_run": ([vp, u32, C.POINTER(_Record), vp], C.c_int),
+        "kcl_sim_plan_classes": ([vp, vp], C.c_int),
+        "kcl_sampling_create": ([vp, u32, C.POINTER(_SamplingOpts), C.POINTER(vp)], C.c_int),
+        "kcl_sampling_destroy": ([vp], None),
+        "kcl_sampling_get_info": ([vp, C.POINTER(_SamplingInfo)], C.c_int),
+        "kcl_sampling_cliques": ([vp, vp], C.c_int),
+        "kcl_sampling_step": ([vp], C.c_int),
+        "kcl_sampling_loads": ([vp, vp], C.c_int),
+        "kcl_sampling_report": ([vp, C.POINTER(_SamplingRecord), vp], C.c_int),
+        "kcl_sampling_run": ([vp, u32, C.POINTER(_SamplingRecord), C.POINTER(u32)], C.c_int),
     }
     for name, (args, res) in sigs.items():
         f = getattr(L, name)
@@ -592,7 +619,7 @@ class Sim:
         """Launch shape of the plan's word classes: vertex-task classes,
         then edge-task classes (only those with tasks)."""
         arr = (_ClassInfo * (2 * NUM_CLASSES))()
-        _check(lib().kcl_sim_plan_classes(self._h, arr))
+        _check(lib().kcl_sim_plan_classes(self._h, C.cast(arr, C.c_void_p)))
         out = []
         for i, c in enumerate(arr):
             if c.count:
@@ -808,6 +835,76 @@ class Trim:
         enc = lambda p: p.encode() if p else None  # noqa: E731
         _check(lib().kcl_trim_write(self._h, enc(ckdeg), enc(ckcore), enc(ckdens), enc(trimmed),
                                     enc(stat), t0_unix))
+
+
+class Sampling:
+    """Sampling++ (kCDensestSamplingKclistpp): k-cliques kept with
+    probability min(1, target / count) by a seeded hash of their ids,
+    synchronous "++" rounds over them, approximate densest subgraphs."""
+
+    def __init__(self, graph: Graph, k: int, target: int = 0, seed: int = 0):
+        self.graph = graph
+        self.k = k
+        o = _SamplingOpts()
+        o.target = target
+        o.seed = seed
+        h = C.c_void_p()
+        _check(lib().kcl_sampling_create(graph.handle, k, C.byref(o), C.byref(h)))
+        self._h = h
+
+    def close(self) -> None:
+        if self._h:
+            lib().kcl_sampling_destroy(self._h)
+            self._h = None
+
+    def __del__(self):  # pragma: no cover - interpreter shutdown order
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def info(self) -> dict:
+        i = _SamplingInfo()
+        _check(lib().kcl_sampling_get_info(self._h, C.byref(i)))
+        return {f: getattr(i, f) for f, _ in _SamplingInfo._fields_}
+
+    def cliques(self) -> np.ndarray:
+        """Kept cliques, (sampled, k) uint32, each row ascending (rows in no
+        particular order)."""
+        n = self.info["sampled"]
+        out = np.zeros((max(n, 1), self.k), np.uint32)
+        _check(lib().kcl_sampling_cliques(self._h, _ptr(out)))
+        return out[:n]
+
+    def step(self) -> None:
+        _check(lib().kcl_sampling_step(self._h))
+
+    def loads(self) -> np.ndarray:
+        out = np.zeros(max(self.graph.n, 1), np.uint64)
+        _check(lib().kcl_sampling_loads(self._h, _ptr(out)))
+        return out[:self.graph.n]
+
+    def report(self, want_order: bool = False):
+        r = _SamplingRecord()
+        order = np.zeros(max(self.graph.n, 1), np.uint32) if want_order else None
+        _check(lib().kcl_sampling_report(self._h, C.byref(r), _ptr(order)))
+        rec = {f: getattr(r, f) for f, _ in _SamplingRecord._fields_}
+        return (rec, order[:self.graph.n]) if want_order else rec
+
+    def run(self, num_iter: int) -> list[dict]:
+        cap = max(1, num_iter.bit_length())
+        recs = (_SamplingRecord * cap)()
+        nrec = C.c_uint32(cap)
+        _check(lib().kcl_sampling_run(self._h, num_iter, recs, C.byref(nrec)))
+        return [{f: getattr(r, f) for f, _ in _SamplingRecord._fields_}
+                for r in recs[:nrec.value]]
 
 
 def exact(graph: Graph, k: int, max_iter: int = 0, max_network_cliques: int = 0,

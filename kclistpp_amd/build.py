@@ -36,11 +36,12 @@ HIP_FLAGS = [
 ]
 
 SOURCES = ["host_io.cpp", "host_order.cpp", "graph.hip", "pass.hip", "plan.hip", "sim.hip", "batch.hip",
-           "costmodel.hip", "trim.cpp", "exact.cpp", "noalloc.hip"]
+           "costmodel.hip", "trim.cpp", "exact.cpp", "noalloc.hip", "sampling.hip"]
 CLIS = {
     "kCDensestCmpSync": "cli_cmpsync.c",
     "kClistCoreTrim": "cli_trim.c",
     "kCDensestMem": "cli_exact.c",
+    "kCDensestSamplingKclistpp": "cli_sampling.c",
 }
 
 

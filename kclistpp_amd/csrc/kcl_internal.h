@@ -108,7 +108,30 @@ enum PassMode : uint32_t {
   kCreditWinner = 0,  // each clique credits its minimum-key member
   kCreditRoot = 1,    // each clique credits its root (per-root clique counts)
   kCreditAll = 2,     // each clique credits every member (k-clique degrees)
+  kEmit = 3,          // each clique kept by the sampler is written out (any k >= 3,
+                      // run by the walk kernels; keys must all be equal)
 };
+
+// Emit passes: where the kept cliques go.  The walk kernels receive the id
+// buffer in place of the count vector and this block in place of the
+// timers.  A clique (ids ascending) is kept iff the hash of its ids with
+// `seed` is below thr53 / 2^53; `count` counts every kept clique, and only
+// the first `cap` are written.
+struct EmitCtl {
+  unsigned long long count;
+  unsigned long long cap;
+  unsigned long long thr53;
+  unsigned long long seed;
+};
+
+// The sampler's hash of a clique (ids in ascending order): splitmix64
+// steps over the ids, the top 53 bits compared with thr53.
+__host__ __device__ inline unsigned long long mix64(unsigned long long z) {
+  z += 0x9e3779b97f4a7c15ull;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
 
 struct PassPlan {
   // Vertex tasks grouped by word class, heaviest first inside each class.

@@ -154,16 +154,20 @@ kcl_status launch_pass_raw(const kcl_graph *g, const uint64_t *d_key, uint32_t k
       const uint32_t count = cnts[c];
       if (!count) continue;
       const bool all = mode == kCreditAll;
+      // Emit passes list cliques with the walk kernels at every k.
+      const bool walk = mode == kEmit || kk > (uint32_t)kMaxTemplK;
       const uint32_t lc = launch_class((uint32_t)c, kk, all);
       if (lc >= kNumClasses)
         return fail(KCL_E_UNSUPPORTED, "k = %u: no word class has room for a %u-level %s walk",
                     kk, kk - 3, all ? "k-clique-degree" : "counting");
-      PassFn fn = kernel_for(lc, kk, all);
+      PassFn fn = mode == kEmit ? pass_table().deep[lc][0] : kernel_for(lc, kk, all);
       if (!fn) return fail(KCL_E_UNSUPPORTED, "no enumeration kernel for k = %u", kk);
-      const uint32_t waves = waves_for(lc, kk, all);
-      // Large-k kernels read the clique size from the bits above the mode.
-      const uint32_t mode_arg =
-          kk > (uint32_t)kMaxTemplK ? ((uint32_t)mode | (kk << 8)) : (uint32_t)mode;
+      const uint32_t waves = walk ? deep_waves(lc) : class_waves(lc, kk, all);
+      // The walk kernels read the clique size from the bits above the mode.
+      const uint32_t mode_arg = walk ? ((uint32_t)mode | (kk << 8)) : (uint32_t)mode;
+      // Emit passes: prof is the emit block, shared by every class.
+      unsigned long long *prof_arg =
+          mode == kEmit ? prof : (prof ? prof + 2 * (size_t)begin[c] : nullptr);
       uint32_t per_cu = blocks_per_cu(lc, props, waves);
       if (plan.max_per_cu && plan.max_per_cu < per_cu) per_cu = plan.max_per_cu;
       uint64_t blocks = (uint64_t)props.cu_count * per_cu;
@@ -176,8 +180,7 @@ kcl_status launch_pass_raw(const kcl_graph *g, const uint64_t *d_key, uint32_t k
                          g->core_h, plan.tasks + begin[c], count,
                          grp ? count : std::min(plan.light_from[c], count), g->dense_items,
                          plan.tickets + 2 * (grp * kNumClasses + c), plan.tickets + kTickets,
-                         plan.list, plan.nlist, mode_arg, d_cnt,
-                         prof ? prof + 2 * (size_t)begin[c] : nullptr);
+                         plan.list, plan.nlist, mode_arg, d_cnt, prof_arg);
       KCL_HIP(hipGetLastError());
     }
   }

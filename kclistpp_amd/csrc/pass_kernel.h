@@ -60,6 +60,10 @@
 // Values one lane holds reach the others through readlane, never through
 // LDS: the compiler orders a lane's LDS accesses only against its own.
 // Every lane writes the same path entry, so each reads back its own write.
+//
+// The walk also lists cliques (mode kEmit, any k >= 3, all keys equal so
+// that ranks follow ids): every clique found is hashed and, when the
+// sampler keeps it, written out with its ids in ascending order.
 #pragma once
 
 #include "kcl_internal.h"
@@ -239,6 +243,25 @@ __device__ __forceinline__ uint32_t wave_edges_from(const unsigned long long (&S
   return c;
 }
 
+// wave_edges_from<W, 0> without the scalar-mask select, for callers whose
+// set the compiler may keep in vector registers (the walk kernels).
+template <int W>
+__device__ __forceinline__ uint32_t wave_edges_plain(const unsigned long long (&S)[W],
+                                                     const unsigned long long *rows,
+                                                     uint32_t lane) {
+  uint32_t c = 0;
+#pragma unroll
+  for (int j = 0; j < W; ++j) {
+    if (S[j] == 0) continue;
+    const unsigned long long *r = rows + (uint32_t)(64 * j + lane) * row_stride<W>();
+    uint32_t cj = 0;
+#pragma unroll
+    for (int jj = j; jj < W; ++jj) cj += (uint32_t)__popcll(S[jj] & r[jj]);
+    c += ((S[j] >> lane) & 1ull) ? cj : 0u;
+  }
+  return c;
+}
+
 // This lane's share of the triangles inside S, for the members w of S in
 // words WI and above: the set of w's later neighbours, S & A'[w], has no
 // member below w's own word, so it is formed (and moved to scalar
@@ -343,6 +366,30 @@ __host__ __device__ constexpr uint32_t deep_words(uint32_t levels, bool all) {
   return (all ? levels : 0u) + (levels + 3u) / 4u;
 }
 
+// Emit passes: the hash of the ids ahead of the last two (the walk's
+// prefix, ascending) and of the whole clique.
+__device__ __forceinline__ unsigned long long emit_hash(unsigned long long h, uint32_t id) {
+  return mix64(h ^ id);
+}
+
+// Emit passes: the slot of this lane's clique (keep = the sampler kept
+// it), one atomic per wavefront; ~0 when it is not to be written (not kept,
+// or past the buffer's capacity).  Called by every lane of the wavefront.
+__device__ __forceinline__ unsigned long long emit_slot(bool keep, uint32_t lane,
+                                                        EmitCtl *__restrict__ ctl,
+                                                        unsigned long long cap) {
+  const unsigned long long m = __ballot(keep);
+  if (m == 0ull) return ~0ull;
+  const uint32_t leader = (uint32_t)__builtin_ctzll(m);
+  unsigned long long base = 0;
+  if (lane == leader) base = atomicAdd(&ctl->count, (unsigned long long)__popcll(m));
+  base = ((unsigned long long)(uint32_t)__shfl((int)(uint32_t)(base >> 32), (int)leader) << 32) |
+         (uint32_t)__shfl((int)(uint32_t)base, (int)leader);
+  if (!keep) return ~0ull;
+  const unsigned long long slot = base + (unsigned long long)__popcll(m & ((1ull << lane) - 1ull));
+  return slot < cap ? slot : ~0ull;
+}
+
 // In-place inclusive scan of a[0..d) by one wavefront.
 __device__ __forceinline__ void wave_scan(uint32_t *a, uint32_t d, uint32_t lane) {
   const uint32_t chunk = (d + 63) >> 6;
@@ -384,9 +431,11 @@ __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restric
   static_assert(NW >= 1 && NW <= 16, "at most 16 wavefronts per workgroup");
   // Large-k kernels take the clique size in the bits above the mode.
   uint32_t kdeep = 0;
+  unsigned long long *timers = prof;
   if constexpr (K == 0) {
     kdeep = mode >> 8;
     mode &= 0xffu;
+    if (mode == kEmit) timers = nullptr;  // prof carries the emit block
   }
   constexpr uint32_t NT = 64 * NW;  // threads of the workgroup sharing one root
   constexpr uint32_t RS = (uint32_t)row_stride<W>();
@@ -435,9 +484,9 @@ __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restric
     const uint32_t ti = chunk_base + (chunk_size - chunk_left);
     --chunk_left;
     if (ti >= ntasks) break;  // same value in every thread of the workgroup
-    // Optional per-task timing (shader clock): prof[2 ti] = cycles until the
-    // rows are built, prof[2 ti + 1] = cycles of the whole task.
-    const unsigned long long t_start = prof ? __builtin_amdgcn_s_memtime() : 0ull;
+    // Optional per-task timing (shader clock): timers[2 ti] = cycles until
+    // the rows are built, timers[2 ti + 1] = cycles of the whole task.
+    const unsigned long long t_start = timers ? __builtin_amdgcn_s_memtime() : 0ull;
     const Task tk = tasks[ti];
     const uint32_t s = tk.s;
     // A vertex task (root s, part of np) or an edge task (root edge (s, u),
@@ -676,25 +725,86 @@ __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restric
         }
       }
       __syncthreads();
-      if (prof && tid == 0) prof[2 * (size_t)ti] = __builtin_amdgcn_s_memtime() - t_start;
+      if (timers && tid == 0) timers[2 * (size_t)ti] = __builtin_amdgcn_s_memtime() - t_start;
 
       if constexpr (K == 0) {
-        // Large k: a wavefront per item u (ranks of this part), the
-        // (kdeep-2)-cliques of A'[u] walked depth first.  c = members still
-        // to choose inside the current set; at c = 3 the chosen member's set
-        // is final and its edges are counted by the lanes.
+        // Large k, and the emit passes: a wavefront per item u (ranks of
+        // this part), the (kdeep-2)-cliques of A'[u] walked depth first.
+        // c = members still to choose inside the current set; at c = 3 the
+        // chosen member's set is final and its edges are counted (or
+        // listed) by the lanes.  Items of kdeep <= 4 (emit passes only)
+        // stop at the item's own set.
         const uint32_t lo = (uint32_t)((uint64_t)part * d / np);
         const uint32_t hi = (uint32_t)((uint64_t)(part + 1) * d / np);
         if (tid == 0) sh.next_item = lo;
         __syncthreads();
         const uint32_t wid = tid >> 6;
-        const uint32_t levels = kdeep - 3;  // path / count slots, 1-based
+        const uint32_t levels = kdeep > 3 ? kdeep - 3 : 1u;  // path / count slots, 1-based
         unsigned long long *ws =
             reinterpret_cast<unsigned long long *>(sh.key) + wid * (Lds<W>::D / NW);
         unsigned long long *lacc = ws;  // [levels] (k-clique degrees; lane 0 only)
         uint16_t *path = reinterpret_cast<uint16_t *>(ws + (ALL ? levels : 0u));  // [levels]
         const bool word_lane = lane < (uint32_t)W;
         const uint32_t nwin = sh.nwin;
+        // Emit passes: the id buffer and the sampler (cnt and prof carry them).
+        const bool emit = !ALL && mode == kEmit;
+        uint32_t *eout = reinterpret_cast<uint32_t *>(cnt);
+        EmitCtl *ectl = reinterpret_cast<EmitCtl *>(prof);
+        unsigned long long ecap = 0, ethr = 0, eseed = 0;
+        uint32_t ksz = kdeep + (edge_task ? 1u : 0u);  // ids per listed clique
+        if (emit) {
+          ecap = ectl->cap;
+          ethr = ectl->thr53;
+          eseed = ectl->seed;
+        }
+        // The hash of a clique ends with the root pair's smaller node (edge
+        // tasks) and the root.
+        auto tail_hash = [&](unsigned long long h) {
+          if (edge_task) h = emit_hash(h, eu);
+          return emit_hash(h, s);
+        };
+        // Emit: write this lane's clique (prefix ranks u, path[1..np-1],
+        // then the ranks x, y when given) if kept.
+        auto write_clique = [&](unsigned long long slot, uint32_t u0, uint32_t npath, uint32_t wl,
+                                bool has_w, uint32_t x, bool has_x, uint32_t y, bool has_y) {
+          if (slot == ~0ull) return;
+          uint32_t *o = eout + slot * ksz;
+          uint32_t at = 0;
+          o[at++] = sh.beg[u0];
+          for (uint32_t i = 1; i < npath; ++i) o[at++] = sh.beg[path[i]];
+          if (has_w) o[at++] = sh.beg[wl];
+          if (has_x) o[at++] = sh.beg[x];
+          if (has_y) o[at++] = sh.beg[y];
+          if (edge_task) o[at++] = eu;
+          o[at] = s;
+        };
+        // The edges of the final set F (scalar registers), whose cliques
+        // also hold the prefix hashed in hp: counted, or listed.
+        auto final_edges = [&](const unsigned long long (&F)[W], unsigned long long hp,
+                               uint32_t u0, uint32_t npath, uint32_t wl, bool has_w) -> uint32_t {
+          if (!emit) return wave_edges_plain<W>(F, sh.rows, lane);
+#pragma unroll
+          for (int j = 0; j < W; ++j) {
+            if (F[j] == 0ull) continue;
+            const bool mine = (F[j] >> lane) & 1ull;
+            const uint32_t y = 64 * (uint32_t)j + lane;
+            const unsigned long long hy = emit_hash(hp, mine ? sh.beg[y] : 0u);
+#pragma unroll
+            for (int jj = j; jj < W; ++jj) {
+              unsigned long long e = mine ? (F[jj] & sh.rows[y * RS + jj]) : 0ull;
+              while (__ballot(e != 0ull)) {
+                const bool has = e != 0ull;
+                const uint32_t z = 64 * (uint32_t)jj + (has ? (uint32_t)__builtin_ctzll(e) : 0u);
+                if (has) e &= e - 1ull;
+                const unsigned long long h = has ? tail_hash(emit_hash(hy, sh.beg[z])) : 0ull;
+                const bool keep = has && (h >> 11) < ethr;
+                write_clique(emit_slot(keep, lane, ectl, ecap), u0, npath, wl, has_w, y, true, z,
+                             true);
+              }
+            }
+          }
+          return 0u;
+        };
         for (;;) {
           uint32_t it = 0;
           if (lane == 0) it = atomicAdd(&sh.next_item, 1u);
@@ -706,6 +816,48 @@ __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restric
           unsigned long long item_total = 0;
           if constexpr (ALL) {
             if (lane == 0) lacc[1] = 0;
+          }
+          if constexpr (!ALL) {
+            if (c <= 2) {
+              // The item's own set is final (kdeep <= 4, emit passes).
+              const unsigned long long hu = emit ? emit_hash(eseed, sh.beg[u]) : 0ull;
+              if (c == 0) {
+                if (emit) {
+                  const bool keep = lane == 0 && (tail_hash(hu) >> 11) < ethr;
+                  write_clique(emit_slot(keep, lane, ectl, ecap), u, 1, 0, false, 0, false, 0,
+                               false);
+                } else {
+                  item_total = 1;
+                }
+              } else {
+                unsigned long long F[W];
+#pragma unroll
+                for (int j = 0; j < W; ++j) F[j] = read_lane64(R, (uint32_t)j);
+                if (c == 1) {
+                  if (emit) {
+#pragma unroll
+                    for (int j = 0; j < W; ++j) {
+                      if (F[j] == 0ull) continue;
+                      const bool mine = (F[j] >> lane) & 1ull;
+                      const uint32_t z = 64 * (uint32_t)j + lane;
+                      const bool keep =
+                          mine && (tail_hash(emit_hash(hu, sh.beg[z])) >> 11) < ethr;
+                      write_clique(emit_slot(keep, lane, ectl, ecap), u, 1, 0, false, z, true, 0,
+                                   false);
+                    }
+                  } else {
+                    item_total = popc_set<W>(F);
+                  }
+                } else {
+                  item_total = wave_sum((unsigned long long)final_edges(F, hu, u, 1, 0, false));
+                }
+              }
+              if (lane == 0 && item_total) {
+                if (!to_root && u < nwin) atomicAdd(&sh.cred[u], item_total);
+                else root_acc += item_total;
+              }
+              continue;
+            }
           }
           for (;;) {
             const unsigned long long nz = __ballot(R != 0ull);
@@ -763,7 +915,13 @@ __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restric
                   ce += cy;
                 }
               } else {
-                ce = wave_edges_from<W, 0>(F, sh.rows, lane);
+                unsigned long long hp = 0;
+                if (emit) {
+                  hp = emit_hash(eseed, sh.beg[u]);
+                  for (uint32_t i = 1; i < l; ++i) hp = emit_hash(hp, sh.beg[path[i]]);
+                  hp = emit_hash(hp, sh.beg[w]);
+                }
+                ce = final_edges(F, hp, u, l, w, true);
               }
               const unsigned long long cb = wave_sum((unsigned long long)ce);
               if constexpr (ALL) {
@@ -1046,7 +1204,7 @@ __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restric
       atomicAdd(&cnt[r], sh.root_sum);
       if (ALL && edge_task) atomicAdd(&cnt[eu], sh.root_sum);
     }
-    if (prof && tid == 0) prof[2 * (size_t)ti + 1] = __builtin_amdgcn_s_memtime() - t_start;
+    if (timers && tid == 0) timers[2 * (size_t)ti + 1] = __builtin_amdgcn_s_memtime() - t_start;
     for (uint32_t i = tid; i < d; i += NT) {
       const unsigned long long c = sh.cred[i];
       if (c) atomicAdd(&cnt[kRanked ? sh.beg[i] : sh.node[i]], c);

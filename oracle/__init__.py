@@ -19,6 +19,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(HERE, "_build", "libkcloracle.so")
 REF_SO = os.path.join(HERE, "_ref", "libkclref.so")
 REF_TRIM_BIN = os.path.join(HERE, "_ref", "kClistCoreTrim")
+REF_SAMPLING_BIN = os.path.join(HERE, "_ref", "kCDensestSamplingKclistpp")
 REF_CMPSYNC_BIN = os.path.join(HERE, "_ref", "kCDensestCmpSync")
 
 _u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
