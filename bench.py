@@ -81,6 +81,44 @@ def profiled(workload: str, key: str, text: bool = False):
     return None
 
 
+def run_reference(cmd: list, cwd: str, timeout_s: float):
+    """Runs a reference program with its output on a pseudo-terminal (so its
+    stdio is line-buffered) and passes each line on as it is printed.
+    Returns (finished, seconds, lines)."""
+    import pty
+    import select
+    import subprocess
+    master, slave = pty.openpty()
+    c0 = time.perf_counter()
+    proc = subprocess.Popen(cmd, cwd=cwd, stdout=slave, stderr=slave, close_fds=True)
+    os.close(slave)
+    buf, lines, killed = b"", [], False
+    while True:
+        ready, _, _ = select.select([master], [], [], 1.0)
+        if ready:
+            try:
+                data = os.read(master, 65536)
+            except OSError:  # the child closed its side
+                data = b""
+            if not data:
+                break
+            buf += data
+            while b"\n" in buf:
+                line, buf = buf.split(b"\n", 1)
+                text = line.decode(errors="replace").rstrip()
+                lines.append(text)
+                log("reference: " + text)
+        elif proc.poll() is not None:
+            break
+        if time.perf_counter() - c0 > timeout_s:
+            proc.kill()
+            killed = True
+            break
+    rc = proc.wait()
+    os.close(master)
+    return (not killed and rc == 0), time.perf_counter() - c0, lines
+
+
 def cpu_model() -> str:
     try:
         for line in open("/proc/cpuinfo"):
@@ -524,27 +562,9 @@ def run_sampling(args, kcl, gg):
         with tempfile.TemporaryDirectory() as td:
             path = os.path.join(td, "edges.txt")
             gg.write_text(pairs, path)
-            c0 = time.perf_counter()
-            # The reference's own lines are passed on as it prints them.
-            proc = subprocess.Popen([REF_SAMPLING_BIN, str(host_threads()), str(T), str(k), path],
-                                    cwd=td, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
-                                    text=True, bufsize=1)
-            lines, ok = [], False
-            try:
-                for line in proc.stdout:
-                    log("reference: " + line.rstrip())
-                    if line.startswith("Approximate"):
-                        lines.append(line.rstrip())
-                    if time.perf_counter() - c0 > args.cpu_timeout:
-                        break
-                ok = proc.wait(timeout=max(1.0, args.cpu_timeout - (time.perf_counter() - c0))) == 0
-            except subprocess.TimeoutExpired:
-                ok = False
-            if proc.poll() is None:
-                proc.kill()
-                proc.wait()
-                ok = False
-            cs = time.perf_counter() - c0
+            ok, cs, out = run_reference([REF_SAMPLING_BIN, str(host_threads()), str(T), str(k),
+                                         path], td, args.cpu_timeout)
+            lines = [x for x in out if x.startswith("Approximate")]
         cpu = {"value": cs, "unit": "s", "cores": host_threads(), "kind": "reference",
                "cpu_model": cpu_model(),
                "sample": f"the reference kCDensestSamplingKclistpp ({host_threads()} OpenMP "
@@ -559,6 +579,79 @@ def run_sampling(args, kcl, gg):
            "config": {"workload": f"{args.workload}: {desc}", "k": k, "T": T},
            "phases_s": {"graph": t1 - t0, "count_sample": t2 - t1, "rounds_reports": t3 - t2},
            "result": {"info": info, "records": recs}, "cpu_baseline": cpu}
+    print(json.dumps(out), flush=True)
+
+
+def run_cmpsync(args, kcl, gg):
+    """kCDensestCmpSync end to end: the text edge file -> reader -> graph ->
+    plan -> T = --steps iterations (rounded down to a power of two) with a
+    report after 1, 2, 4, ... -> the stat file; wall clock over all of it.
+    cpu_baseline: the reference program on the same file and host threads
+    when it finishes within --cpu-timeout, else (kind "reference") its
+    measured load time plus the run's passes priced at one measured
+    CDF_CliqueEnum(FRANK_WOLFE) pass of the reference on all threads."""
+    import subprocess
+    from oracle import REF_CMPSYNC_BIN, RefHarness
+    wname, seed, sd, k, desc = WORKLOADS[args.workload]
+    T = args.steps
+    pairs = workload_pairs(args, gg, wname, seed, sd)
+    model = load_model(args, kcl)
+    kcl.device_info()  # the HIP runtime starts here, outside the timed region
+    with tempfile.TemporaryDirectory() as td:
+        path = os.path.join(td, "edges.txt")
+        gg.write_text(pairs, path)
+        del pairs
+        t0 = time.perf_counter()
+        pr, n_in = kcl.read_edge_list(path)
+        t1 = time.perf_counter()
+        with kcl.Graph(pr, n_in) as g:
+            del pr
+            t2 = time.perf_counter()
+            recs = kcl.cmpsync(g, k, T, stat_path=os.path.join(td, "stat.txt"),
+                               schedule=kcl.SCHEDULES[args.schedule], model=model)
+            t3 = time.perf_counter()
+        log(f"bench cmpsync: read {t1 - t0:.2f}s, graph {t2 - t1:.2f}s, run {t3 - t2:.2f}s; "
+            f"last record {recs[-1]}")
+        cpu = None
+        if not args.no_cpu_baseline:
+            threads = host_threads()
+            ok, cs, _ = run_reference([REF_CMPSYNC_BIN, str(threads), str(T), str(k), path, "b"],
+                                      td, args.cpu_timeout)
+            if ok:
+                cpu = {"value": cs, "unit": "s", "cores": threads, "kind": "reference",
+                       "cpu_model": cpu_model(),
+                       "sample": f"the reference kCDensestCmpSync ({threads} OpenMP threads) on "
+                                 f"the same edge file, T = {T}: {cs:.2f}s"}
+            else:
+                ref = RefHarness()
+                ref.threads(threads)
+                l0 = time.perf_counter()
+                h = ref.load(path)
+                load_s = time.perf_counter() - l0
+                ref.start(h, k)
+                e = ref.info(h)["e"]
+                pass_s, _ = ref.time_fw_roots(h, np.arange(e, dtype=np.uint32))
+                ref.free(h)
+                iters = 1 << (T.bit_length() - 1)
+                passes = iters + iters.bit_length()  # FW passes, one PAVA pass per report
+                est = load_s + passes * pass_s
+                cpu = {"value": est, "unit": "s", "cores": threads, "kind": "reference",
+                       "cpu_model": cpu_model(),
+                       "sample": f"the reference kCDensestCmpSync did not finish within "
+                                 f"{args.cpu_timeout:.0f}s; estimate = its ReadEdgeList+SortByCore"
+                                 f"+MakeGraph {load_s:.2f}s + {passes} passes x {pass_s:.3f}s "
+                                 f"(one CDF_CliqueEnum(FRANK_WOLFE) pass over all {e} root "
+                                 f"edges, {threads} threads)"}
+            log(f"bench cmpsync: reference {cpu['value']:.2f}s ({cpu['sample']})")
+    if model is not None:
+        model.close()
+    out = {"metric": f"kCDensestCmpSync end to end (k={k}, T={T})", "value": t3 - t0,
+           "unit": "s", "n_gpus": 1, "higher_is_better": False, "vs_baseline": None,
+           "dtype": "u64", "data": "synthetic (seeded generator, kclistpp_amd/graphgen.py)",
+           "config": {"workload": f"{args.workload}: {desc}", "k": k, "T": T,
+                      "schedule": args.schedule},
+           "phases_s": {"read": t1 - t0, "graph": t2 - t1, "plan_iterations_reports": t3 - t2},
+           "records": recs, "cpu_baseline": cpu}
     print(json.dumps(out), flush=True)
 
 
@@ -628,11 +721,12 @@ def main():
     ap.add_argument("--exact-budget", type=int, default=240,
                     help="exact mode: seconds before giving up on a certificate")
     ap.add_argument("--mode", default="sim",
-                    choices=["sim", "exact", "trim", "makespan", "sampling"],
+                    choices=["sim", "exact", "trim", "makespan", "sampling", "cmpsync"],
                     help="sim: visits/s of one iteration (default); exact: time-to-exact; "
                          "trim: k-clique core trimming time; makespan: each schedule's "
                          "pass against a makespan lower bound; sampling: Sampling++ time "
-                         "(--steps rounds)")
+                         "(--steps rounds); cmpsync: kCDensestCmpSync end to end "
+                         "(T = --steps)")
     ap.add_argument("--gpu-peel-min", type=int, default=0,
                     help="trim mode: pops through at least this many cliques go to the GPU "
                          "(0: the library's default)")
@@ -672,6 +766,9 @@ def main():
         return
     if args.mode == "sampling":
         run_sampling(args, kcl, gg)
+        return
+    if args.mode == "cmpsync":
+        run_cmpsync(args, kcl, gg)
         return
     if args.workload == "batch512":
         run_batch(args, kcl, gg)

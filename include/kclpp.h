@@ -486,7 +486,9 @@ typedef struct kcl_exact_opts {
                                     host memory per clique; at most
                                     (2^32 - 1) / k) */
   uint32_t schedule;             /* kcl_schedule of the Sim iterations */
-  uint32_t reserved0;
+  uint32_t gpu_list_min;         /* a certificate region holding at least this many
+                                    cliques has them listed on the GPU, a smaller
+                                    one on the host; 0 = 1,048,576 */
   const kcl_cost_model *model;   /* for KCL_SCHED_LEARNED (nullable) */
   uint32_t reserved[2];
 } kcl_exact_opts;

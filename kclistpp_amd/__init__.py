@@ -105,7 +105,7 @@ class _TrimOpts(C.Structure):
 class _ExactOpts(C.Structure):
     _fields_ = [("max_iter", C.c_uint32), ("max_seconds", C.c_uint32),
                 ("max_network_cliques", C.c_uint64), ("schedule", C.c_uint32),
-                ("reserved0", C.c_uint32), ("model", C.c_void_p), ("reserved", C.c_uint32 * 2)]
+                ("gpu_list_min", C.c_uint32), ("model", C.c_void_p), ("reserved", C.c_uint32 * 2)]
 
 
 class _ExactResult(C.Structure):
@@ -909,7 +909,8 @@ class Sampling:
 
 def exact(graph: Graph, k: int, max_iter: int = 0, max_network_cliques: int = 0,
           stat_path: str | None = None, t0_unix: int = 0, max_seconds: int = 0,
-          schedule: int = SCHED_COUNTED, model: CostModel | None = None):
+          schedule: int = SCHED_COUNTED, model: CostModel | None = None,
+          gpu_list_min: int = 0):
     """Exact k-clique densest subgraph (kCDensestMem's result).  Returns
     (result dict, input ids of the maximal densest subgraph)."""
     o = _ExactOpts()
@@ -917,6 +918,7 @@ def exact(graph: Graph, k: int, max_iter: int = 0, max_network_cliques: int = 0,
     o.max_network_cliques = max_network_cliques
     o.max_seconds = max_seconds
     o.schedule = schedule
+    o.gpu_list_min = gpu_list_min
     o.model = model.handle if model is not None else None
     r = _ExactResult()
     nodes = np.zeros(max(graph.n, 1), np.uint32)

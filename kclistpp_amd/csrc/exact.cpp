@@ -314,8 +314,8 @@ struct Verdict {
 // rounds without one, the flow settles the question instead).
 kcl_status certify(const kcl_graph *g, uint32_t k, uint64_t p, uint64_t q,
                    const std::vector<uint8_t> &cand, const std::vector<double> &b,
-                   uint64_t budget, bool flow_unstable, uint32_t *flows, uint32_t *region_nodes,
-                   Verdict *out) {
+                   uint64_t budget, uint64_t gpu_list_min, bool flow_unstable, uint32_t *flows,
+                   uint32_t *region_nodes, Verdict *out) {
   const uint32_t n = g->n;
   bool first = true;
   std::vector<uint32_t> cliques;  // the last region's cliques (global ids)
@@ -364,7 +364,8 @@ kcl_status certify(const kcl_graph *g, uint32_t k, uint64_t p, uint64_t q,
       return KCL_OK;
     }
     if (!listed) {
-      list_cliques(g, k, alive, &cliques);
+      if (cl >= gpu_list_min && k >= 3) KCL_TRY(list_cliques_gpu(g, k, alive, &cliques));
+      else list_cliques(g, k, alive, &cliques);
       listed = true;
     } else {
       // The region shrank (a denser candidate): keep the cliques inside it.
@@ -457,8 +458,10 @@ extern "C" kcl_status kcl_exact_run(kcl_graph *g, uint32_t k, const kcl_exact_op
       o.max_network_cliques = kcl::kMaxNetworkCliques / k;
     o.max_seconds = opts->max_seconds;
     o.schedule = opts->schedule;
+    o.gpu_list_min = opts->gpu_list_min;
     o.model = opts->model;
   }
+  const uint64_t gpu_list_min = o.gpu_list_min ? o.gpu_list_min : (1ull << 20);
   memset(res, 0, sizeof(*res));
   const auto c0 = std::chrono::steady_clock::now();
   if (t0_unix <= 0) t0_unix = (int64_t)time(nullptr);
@@ -531,7 +534,7 @@ extern "C" kcl_status kcl_exact_run(kcl_graph *g, uint32_t k, const kcl_exact_op
     // tested this round (its region is a superset, and computing the region
     // costs k-clique degree passes over the graph).
     if (best_p <= o.max_network_cliques) {
-      st = kcl::certify(g, k, best_p, best_q, best, b, o.max_network_cliques,
+      st = kcl::certify(g, k, best_p, best_q, best, b, o.max_network_cliques, gpu_list_min,
                         T >= kcl::kFlowUnstableFrom, &flows, &region, &v);
       degree_passes += v.degree_passes;
       host_seconds += v.host_seconds;

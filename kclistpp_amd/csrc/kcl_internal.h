@@ -259,6 +259,10 @@ kcl_status make_plan(kcl_graph *g, uint32_t k, uint32_t split, hipStream_t st, P
 kcl_status plan_for_schedule(kcl_graph *g, uint32_t k, uint32_t schedule,
                              const kcl_cost_model *model, uint32_t split,
                              const std::vector<uint64_t> *keep, hipStream_t st, PassPlan *out);
+// Every k-clique (k >= 3) inside the node set `in` (n flags) of g, k global
+// ids each, listed on the GPU (sampling.hip).
+kcl_status list_cliques_gpu(const kcl_graph *g, uint32_t k, const std::vector<uint8_t> &in,
+                            std::vector<uint32_t> *out);
 // Cost-model features of every node as a root (n rows of KCL_CM_FEATURES).
 kcl_status root_features(kcl_graph *g, uint32_t k, hipStream_t st, std::vector<float> *out);
 // Cost-model predictions for device-resident rows (log2(1 + cycles)).

@@ -30,6 +30,7 @@
 #include <cstring>
 #include <vector>
 
+#include "clique_flow.h"
 #include "kcl_internal.h"
 
 struct kcl_sampling {
@@ -109,7 +110,111 @@ kcl_status count_cliques_of(const std::vector<uint32_t> &pairs, uint32_t n_in, u
   return st;
 }
 
+// Lists the kept cliques of g (keep rate thr53 / 2^53, seed) into a
+// device buffer; *kept = the cliques kept (the buffer holds cap of them; a
+// second listing into an exact-size buffer follows when more were kept).
+kcl_status emit_cliques(kcl_graph *g, uint32_t k, const PassPlan &plan, uint64_t total,
+                        unsigned long long thr53, uint64_t seed, DevBuf<uint32_t> *buf,
+                        uint64_t *kept) {
+  *kept = 0;
+  if (!total) return KCL_OK;
+  DevicePlan dp;
+  KCL_TRY(upload_plan(plan, &dp));
+  const uint32_t n = g->n;
+  DevBuf<unsigned long long> zkey;
+  DevBuf<EmitCtl> ctl;
+  KCL_TRY(zkey.alloc(n));
+  KCL_TRY(zero_u64(zkey.p, n, g->stream));
+  KCL_TRY(ctl.alloc(1));
+  // Room for the expected sample plus seven standard deviations.
+  const double p = thr53 >= (1ull << 53) ? 1.0 : std::ldexp((double)thr53, -53);
+  const double mean = (double)total * p;
+  uint64_t cap = p >= 1.0 ? total : (uint64_t)(mean + 7.0 * std::sqrt(mean) + 1024.0);
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    if (cap > (1ull << 40) / k)
+      return fail(KCL_E_UNSUPPORTED, "%llu kept %u-cliques: over 2^40 ids",
+                  (unsigned long long)cap, k);
+    KCL_TRY(buf->alloc(cap ? cap * k : 1));
+    EmitCtl h{0ull, (unsigned long long)cap, thr53, (unsigned long long)seed};
+    KCL_HIP(hipMemcpyAsync(ctl.p, &h, sizeof(h), hipMemcpyHostToDevice, g->stream));
+    KCL_TRY(launch_pass(g, (const uint64_t *)zkey.p, k, kEmit, dp,
+                        reinterpret_cast<unsigned long long *>(buf->p), g->stream,
+                        reinterpret_cast<unsigned long long *>(ctl.p)));
+    KCL_HIP(hipMemcpyAsync(&h, ctl.p, sizeof(h), hipMemcpyDeviceToHost, g->stream));
+    KCL_TRY(pass_status(dp, g->stream));  // synchronises the stream
+    *kept = h.count;
+    if (h.count <= cap) {
+      if (p >= 1.0 && h.count != total)
+        return fail(KCL_E_HIP, "listed %llu of %llu cliques", (unsigned long long)h.count,
+                    (unsigned long long)total);
+      return KCL_OK;
+    }
+    cap = h.count;
+  }
+  return fail(KCL_E_HIP, "the kept cliques changed between two listings");
+}
+
 }  // namespace
+
+// Every k-clique (k >= 3) inside the node set `in` of g, k global ids each
+// (ascending within a clique, cliques in no particular order): the induced
+// subgraph is built and its cliques listed by the walk kernels, then mapped
+// back to g's ids.
+kcl_status list_cliques_gpu(const kcl_graph *g, uint32_t k, const std::vector<uint8_t> &in,
+                            std::vector<uint32_t> *out) {
+  out->clear();
+  const uint32_t n = g->n;
+  std::vector<uint32_t> local(n, UINT32_MAX), global;
+  for (uint32_t v = 0; v < n; ++v)
+    if (in[v]) {
+      local[v] = (uint32_t)global.size();
+      global.push_back(v);
+    }
+  std::vector<uint32_t> pairs;
+  for (uint32_t v : global)
+    for (uint32_t j = g->cd[v]; j < g->cd[v + 1]; ++j) {
+      const uint32_t x = g->adj[j];
+      if (local[x] == UINT32_MAX) continue;
+      pairs.push_back(local[v]);
+      pairs.push_back(local[x]);
+    }
+  if (pairs.empty() || k < 3) return k < 3 ? fail(KCL_E_INVALID_ARG, "k = %u", k) : KCL_OK;
+  kcl_graph *sub = nullptr;
+  KCL_TRY(kcl_graph_create(pairs.data(), pairs.size() / 2, (uint32_t)global.size(), &sub));
+  kcl_status st = KCL_OK;
+  do {
+    if (k > sub->core + 1) break;
+    PassPlan plan;
+    kcl_plan_info info{};
+    if ((st = make_plan(sub, k, 1u << 17, sub->stream, &plan, nullptr, &info,
+                        KCL_SCHED_DEGREE)) != KCL_OK)
+      break;
+    DevBuf<uint32_t> buf;
+    uint64_t kept = 0;
+    if ((st = emit_cliques(sub, k, plan, info.cliques, 1ull << 53, 0, &buf, &kept)) != KCL_OK)
+      break;
+    out->resize((size_t)kept * k);
+    if (kept) {
+      hipError_t e = hipMemcpyAsync(out->data(), buf.p, out->size() * 4, hipMemcpyDeviceToHost,
+                                    sub->stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(sub->stream);
+      if (e != hipSuccess) {
+        st = fail(KCL_E_HIP, "hipMemcpy of the listed cliques: %s", hipGetErrorString(e));
+        break;
+      }
+    }
+    // The subgraph's ids (its own degeneracy order) back to g's.
+    std::vector<uint32_t> back(sub->n, UINT32_MAX);
+    for (uint32_t i = 0; i < (uint32_t)global.size(); ++i)
+      if (sub->rank[i] < sub->n) back[sub->rank[i]] = global[i];
+    uint32_t *ids = out->data();
+    parallel_chunks(out->size(), host_workers(), [&](uint32_t, uint64_t lo, uint64_t hi) {
+      for (uint64_t i = lo; i < hi; ++i) ids[i] = back[ids[i]];
+    });
+  } while (0);
+  kcl_graph_destroy(sub);
+  return st;
+}
 
 }  // namespace kcl
 
@@ -152,39 +257,10 @@ extern "C" kcl_status kcl_sampling_create(kcl_graph *g, uint32_t k, const kcl_sa
   s->prob = s->total > s->opts.target ? (double)s->opts.target / (double)s->total : 1.0;
   const unsigned long long thr53 =
       s->prob >= 1.0 ? (1ull << 53) : (unsigned long long)std::ldexp(s->prob, 53);
-  DevicePlan dp;
-  if ((st = upload_plan(plan, &dp)) != KCL_OK) return bail(st);
-  DevBuf<unsigned long long> zkey;
-  DevBuf<EmitCtl> ctl;
-  if ((st = zkey.alloc(n)) != KCL_OK) return bail(st);
-  if ((st = zero_u64(zkey.p, n, g->stream)) != KCL_OK) return bail(st);
-  if ((st = ctl.alloc(1)) != KCL_OK) return bail(st);
-  // Room for the expected sample plus seven standard deviations; a larger
-  // sample is listed again into a buffer of its exact size.
-  const double mean = (double)s->total * s->prob;
-  uint64_t cap = s->prob >= 1.0 ? s->total
-                                : (uint64_t)(mean + 7.0 * std::sqrt(mean) + 1024.0);
-  for (int attempt = 0; attempt < 2; ++attempt) {
-    if (cap > (1ull << 40) / k)
-      return bail(fail(KCL_E_UNSUPPORTED, "%llu kept %u-cliques: over 2^40 ids",
-                       (unsigned long long)cap, k));
-    if ((st = s->ck.alloc(cap ? cap * k : 1)) != KCL_OK) return bail(st);
-    EmitCtl h{0ull, (unsigned long long)cap, thr53, (unsigned long long)s->opts.seed};
-    if (hipMemcpyAsync(ctl.p, &h, sizeof(h), hipMemcpyHostToDevice, g->stream) != hipSuccess)
-      return bail(fail(KCL_E_HIP, "hipMemcpy of the sampler"));
-    if ((st = launch_pass(g, (const uint64_t *)zkey.p, k, kEmit, dp,
-                          reinterpret_cast<unsigned long long *>(s->ck.p), g->stream,
-                          reinterpret_cast<unsigned long long *>(ctl.p))) != KCL_OK)
-      return bail(st);
-    if (hipMemcpyAsync(&h, ctl.p, sizeof(h), hipMemcpyDeviceToHost, g->stream) != hipSuccess)
-      return bail(fail(KCL_E_HIP, "hipMemcpy of the sampler"));
-    if ((st = pass_status(dp, g->stream)) != KCL_OK) return bail(st);
-    s->sampled = h.count;
-    if (h.count <= cap) break;
-    if (attempt == 1)
-      return bail(fail(KCL_E_HIP, "the kept cliques changed between two listings"));
-    cap = h.count;
-  }
+  uint64_t kept = 0;
+  if ((st = emit_cliques(g, k, plan, s->total, thr53, s->opts.seed, &s->ck, &kept)) != KCL_OK)
+    return bail(st);
+  s->sampled = kept;
   *out = s;
   return KCL_OK;
 }

@@ -21,6 +21,7 @@ REF_SO = os.path.join(HERE, "_ref", "libkclref.so")
 REF_TRIM_BIN = os.path.join(HERE, "_ref", "kClistCoreTrim")
 REF_SAMPLING_BIN = os.path.join(HERE, "_ref", "kCDensestSamplingKclistpp")
 REF_CMPSYNC_BIN = os.path.join(HERE, "_ref", "kCDensestCmpSync")
+REF_CMPSYNC_BIN = os.path.join(HERE, "_ref", "kCDensestCmpSync")
 
 _u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
 _u64p = np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")

@@ -89,14 +89,17 @@ def test_exact_returns_the_maximal_densest_subgraph(kcl):
     assert r["certified"] and r["n"] == 14 and r["cliques"] == 2 * math.comb(7, 4)
 
 
+# gpu_list_min: 0 lists a certificate region's cliques on the host when it
+# holds fewer than 2^20, 1 always on the GPU (the walk kernels' listing).
+@pytest.mark.parametrize("gpu_list_min", [0, 1])
 @pytest.mark.parametrize("case,k", [("sim_er2k_k12_k4_T8", 4), ("sim_cl3k_k4_T8", 4),
                                     ("sim_cl3k_k5_T4", 5), ("sim_cl1k_dense_k4_T4", 4)])
-def test_exact_vs_oracle_max_flow(kcl, oracle, case, k):
+def test_exact_vs_oracle_max_flow(kcl, oracle, case, k, gpu_list_min):
     gd = load_golden(case)
     pairs = gd["pairs"]
-    _say("exact", case, k)
+    _say("exact", case, k, "gpu_list_min", gpu_list_min)
     with kcl.Graph(pairs) as g:
-        r, nodes = kcl.exact(g, k)
+        r, nodes = kcl.exact(g, k, gpu_list_min=gpu_list_min)
     assert r["certified"] == 1
     og = oracle.dag(pairs)
     rank = og.rank
@@ -189,14 +192,16 @@ def test_exact_planted_clique_by_criterion_a(kcl):
     assert r["n"] == 30 and r["cliques"] == math.comb(30, 4) and r["m"] == 435
 
 
-def test_exact_community_by_max_flow(kcl, oracle):
+@pytest.mark.parametrize("gpu_list_min", [0, 1])
+def test_exact_community_by_max_flow(kcl, oracle, gpu_list_min):
     """A 60-node community with p = 0.8 planted in a Chung-Lu graph, k = 5:
     not a clique, so the certificate is the max-flow; the result's clique
-    count is checked against the oracle's."""
+    count is checked against the oracle's, with the region's cliques listed
+    on the host and on the GPU."""
     pairs, rng = gg.chung_lu(20000, 120000, 2.3, 300.0, 31)
     pairs = gg.finalize(gg.plant_community(pairs, 20000, 60, 0.8, rng), rng)
     with kcl.Graph(pairs) as g:
-        r, nodes = kcl.exact(g, 5)
+        r, nodes = kcl.exact(g, 5, gpu_list_min=gpu_list_min)
     assert r["certified"] == 1 and r["stop"] == 2 and r["max_flow_calls"] >= 1
     og = oracle.dag(pairs)
     mask = np.zeros(og.n, np.uint8)
