@@ -76,9 +76,13 @@ constexpr uint32_t kClassWaves[kNumClasses] = {1, 2, 8, 8, 8, 8, 8, 8, 8};
 // the wide ones (one or two workgroups per CU by LDS) use eight up to k = 7
 // and four from k = 8, where eight would not leave their deeper loops the
 // registers.  Wide means 512+ members for the counting kernels (dense-only
-// from k = 6, few registers) and 640+ for the k-clique-degree kernels.
+// from k = 6, few registers) and 640+ for the k-clique-degree kernels.  The
+// 10-word counting kernels at k = 6, 7 (two workgroups per CU by LDS) run
+// twelve: 24 waves per CU, within the SGPR file (4M/60M k = 6 pass 8 %
+// faster than with eight).
 constexpr uint32_t class_waves(uint32_t c, uint32_t k, bool all = false) {
-  return kClassWords[c] <= 2 ? kClassWaves[c]
+  return (!all && kClassWords[c] == 10 && k >= 6 && k <= 7) ? 12u
+         : kClassWords[c] <= 2 ? kClassWaves[c]
          : kClassWords[c] >= (all ? 10u : 8u) ? (k >= 8 ? 4u : 8u)
                                               : (k <= 5 ? 8u : 4u);
 }
