@@ -362,8 +362,9 @@ typedef struct kcl_batch_info {
   uint64_t edges;
   uint64_t cliques;   /* k-cliques over all graphs (each with its own k) */
   uint32_t tasks;     /* workgroup tasks per iteration */
-  uint32_t row_bits;  /* chosen row-width floor */
-  uint32_t per_cu;    /* chosen cap (0: LDS-limited) */
+  uint32_t row_bits;  /* chosen row-width floor (tune = 1: each k's plan is tuned on its
+                         own; this is the choice for the plan whose pass is longest) */
+  uint32_t per_cu;    /* chosen cap (0: LDS-limited), the same plan's */
   uint32_t reserved;
   double plan_ms;     /* union + plans */
   double tune_ms;     /* autotuning */

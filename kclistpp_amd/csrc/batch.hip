@@ -219,28 +219,44 @@ kcl_status launch_all(kcl_batch *b, const uint64_t *key) {
   return KCL_OK;
 }
 
-// Plans with the row-width floor and the per-CU cap applied.
-kcl_status apply_config(kcl_batch *b, uint32_t min_class, uint32_t per_cu) {
-  for (auto &kp : b->plans) {
-    PassPlan p = kp->host;
-    promote_plan(&p, min_class);
-    KCL_TRY(upload_plan(p, &kp->dev));
-    kp->dev.max_per_cu = per_cu;
-  }
+// One plan with a row-width floor and a per-CU cap applied.
+kcl_status apply_plan_config(kcl_batch::KPlan *kp, uint32_t min_class, uint32_t per_cu) {
+  PassPlan p = kp->host;
+  promote_plan(&p, min_class);
+  KCL_TRY(upload_plan(p, &kp->dev));
+  kp->dev.max_per_cu = per_cu;
   return KCL_OK;
 }
 
-// Time of one pass of every plan (ms, best of two), state unchanged.
-kcl_status time_pass(kcl_batch *b, float *ms) {
+// Every plan with the same floor and cap.
+kcl_status apply_config(kcl_batch *b, uint32_t min_class, uint32_t per_cu) {
+  for (auto &kp : b->plans) KCL_TRY(apply_plan_config(kp.get(), min_class, per_cu));
+  return KCL_OK;
+}
+
+// Time of one pass of every plan, or of plan `only` alone (ms, best of
+// two), state unchanged.
+kcl_status time_pass(kcl_batch *b, float *ms, const kcl_batch::KPlan *only = nullptr) {
   hipStream_t st = b->u->stream;
   float best = 1e30f;
   for (int rep = 0; rep < 2; ++rep) {
     KCL_HIP(hipEventRecord(b->ev[0], st));
-    KCL_TRY(launch_all(b, (const uint64_t *)b->rho.p));
+    if (only)
+      KCL_TRY(launch_pass(b->u, (const uint64_t *)b->rho.p, only->k, kCreditWinner,
+                          const_cast<kcl_batch::KPlan *>(only)->dev, b->cnt.p, st));
+    else
+      KCL_TRY(launch_all(b, (const uint64_t *)b->rho.p));
     KCL_HIP(hipEventRecord(b->ev[1], st));
     KCL_TRY(zero_u64(b->cnt.p, b->nodes(), st));
     std::vector<uint32_t> status;
-    KCL_TRY(enqueue_status(b, status, st));
+    if (only) {
+      status.assign(1, 0);
+      KCL_HIP(hipMemcpyAsync(status.data(),
+                             pass_status_word(const_cast<kcl_batch::KPlan *>(only)->dev), 4,
+                             hipMemcpyDeviceToHost, st));
+    } else {
+      KCL_TRY(enqueue_status(b, status, st));
+    }
     KCL_HIP(hipStreamSynchronize(st));
     KCL_TRY(check_status(status));
     float t = 0;
@@ -251,26 +267,41 @@ kcl_status time_pass(kcl_batch *b, float *ms) {
   return KCL_OK;
 }
 
-// Row-width floors 64/128/256 bits (classes 0, 1, 3) and resident
-// workgroups per CU (LDS-limited, 8, 16); the fastest pass wins.
+// Per plan (one per k: each has its own kernels and task mix), row-width
+// floors 64/128/256 bits (classes 0, 1, 3) and resident workgroups per CU
+// (LDS-limited, 8, 16), each plan timed alone; the fastest configuration of
+// each plan wins.  The plans' passes run nearly one after another (each
+// fills the device), so their best configurations add up.  Reports the
+// configuration of the plan whose best pass is longest.
 kcl_status autotune(kcl_batch *b, uint32_t *min_class, uint32_t *per_cu) {
   static const uint32_t kFloors[] = {0, 1, 3};
   static const uint32_t kCaps[] = {0, 8, 16};
-  float best = 1e30f;
   *min_class = 0;
   *per_cu = 0;
-  for (uint32_t f : kFloors)
-    for (uint32_t c : kCaps) {
-      KCL_TRY(apply_config(b, f, c));
-      float ms = 0;
-      KCL_TRY(time_pass(b, &ms));
-      if (ms < best) {
-        best = ms;
-        *min_class = f;
-        *per_cu = c;
+  KCL_TRY(apply_config(b, 0, 0));  // every plan on the device before any is timed
+  float longest = -1.0f;
+  for (auto &kp : b->plans) {
+    float best = 1e30f;
+    uint32_t bf = 0, bc = 0;
+    for (uint32_t f : kFloors)
+      for (uint32_t c : kCaps) {
+        KCL_TRY(apply_plan_config(kp.get(), f, c));
+        float ms = 0;
+        KCL_TRY(time_pass(b, &ms, kp.get()));
+        if (ms < best) {
+          best = ms;
+          bf = f;
+          bc = c;
+        }
       }
+    KCL_TRY(apply_plan_config(kp.get(), bf, bc));
+    if (best > longest) {
+      longest = best;
+      *min_class = bf;
+      *per_cu = bc;
     }
-  return apply_config(b, *min_class, *per_cu);
+  }
+  return KCL_OK;
 }
 
 kcl_status capture(kcl_batch *b) {
