@@ -39,6 +39,10 @@ WORKLOADS = {
                 "Chung-Lu n=1e6 m=1.6e7 gamma=2.3 w_max=2e4 + 150-vertex community p=0.8, k=6"),
     "cl4m": ("cl4m", 1, 1, 6,
              "Chung-Lu n=4e6 m=6e7 gamma=2.1 w_max=4e4 + 200-vertex near-clique p=0.9, k=6"),
+    "rmat15": ("rmat23", 1, 256, 7,
+               "R-MAT (0.57, 0.19, 0.19, 0.05) scale 15, m=2e8/256 before dedup, k=7"),
+    "rmat16": ("rmat23", 1, 128, 7,
+               "R-MAT (0.57, 0.19, 0.19, 0.05) scale 16, m=2e8/128 before dedup, k=7"),
 }
 
 
@@ -499,8 +503,15 @@ def run_makespan(args, kcl, gg):
     rho = main.rho()
     t += 1
     out = {}
-    for name in ("degree", "counted", "learned"):
-        s = main if name == "learned" else kcl.Sim(g, k, schedule=kcl.SCHEDULES[name])
+    retrained = None
+    for name in ("degree", "counted", "learned", "online"):
+        if name == "online":
+            # The shipped model fine-tuned on this graph's own pass timings
+            # (the learned plan's profiled pass at this state), then replanned.
+            from kclistpp_amd import train_cost_model as tcm
+            main.set_rho(rho)
+            retrained = tcm.online(main, model, epochs=10)
+        s = main if name in ("learned", "online") else kcl.Sim(g, k, schedule=kcl.SCHEDULES[name])
         ms, counts = [], None
         for _ in range(max(args.steps, 1)):
             s.set_rho(rho)
@@ -525,12 +536,15 @@ def run_makespan(args, kcl, gg):
             s.close()
     main.close()
     model.close()
+    if retrained is not None:
+        retrained.close()
     lb = min(v["lb_ms"] for v in out.values())
     res = {"metric": "pass makespan vs lower bound, per schedule", "unit": "ms",
            "workload": f"{args.workload}: {desc}", "k": k, "t": t, "clock_mhz": clock_hz / 1e6,
            "cu_count": cus, "schedules": out, "lb_min_ms": lb,
            "plain_over_lb_min": out["degree"]["pass_ms"] / lb if lb > 0 else None,
-           "learned_vs_plain": out["degree"]["pass_ms"] / out["learned"]["pass_ms"]}
+           "learned_vs_plain": out["degree"]["pass_ms"] / out["learned"]["pass_ms"],
+           "online_vs_plain": out["degree"]["pass_ms"] / out["online"]["pass_ms"]}
     g.close()
     print(json.dumps(res), flush=True)
 

@@ -32,9 +32,11 @@
 // member) and are flushed with one global atomicAdd per non-zero member,
 // plus one wave-reduced atomicAdd for the root.
 //
-// Heavy roots are split into parts: part p of np takes items
-// [p*I/np, (p+1)*I/np) and rebuilds A' (the rebuild is cheap next to the
-// counting it splits).
+// Heavy roots are split into parts: part p of np takes items p, p + np,
+// p + 2 np, ... and rebuilds A' (the rebuild is cheap next to the counting
+// it splits).  Strided parts see the same mix of items: in ranked kernels
+// an item's cost grows steeply with its members' ranks, so contiguous
+// ranges would leave one part with most of the root's work.
 //
 // Dense roots (k >= 6).  The k >= 6 kernels of rows of 4+ words number
 // the members by their (key, id) rank before building A', so A'[x] holds only ranks above x
@@ -589,9 +591,7 @@ __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restric
     unsigned long long root_acc = 0;
     if constexpr (K == 2) {
       // Cliques {s, u}: items are the members themselves.
-      const uint32_t lo = (uint32_t)((uint64_t)part * d / np);
-      const uint32_t hi = (uint32_t)((uint64_t)(part + 1) * d / np);
-      for (uint32_t u = lo + tid; u < hi; u += NT) {
+      for (uint32_t u = part + tid * np; u < d; u += NT * np) {
         if constexpr (ALL) {
           sh.cred[u] += 1;
           root_acc += 1;
@@ -734,9 +734,7 @@ __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restric
         // chosen member's set is final and its edges are counted (or
         // listed) by the lanes.  Items of kdeep <= 4 (emit passes only)
         // stop at the item's own set.
-        const uint32_t lo = (uint32_t)((uint64_t)part * d / np);
-        const uint32_t hi = (uint32_t)((uint64_t)(part + 1) * d / np);
-        if (tid == 0) sh.next_item = lo;
+        if (tid == 0) sh.next_item = 0;  // the j-th item of this part is part + j np
         __syncthreads();
         const uint32_t wid = tid >> 6;
         const uint32_t levels = kdeep > 3 ? kdeep - 3 : 1u;  // path / count slots, 1-based
@@ -809,8 +807,8 @@ __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restric
           uint32_t it = 0;
           if (lane == 0) it = atomicAdd(&sh.next_item, 1u);
           it = (uint32_t)__builtin_amdgcn_readfirstlane((int)it);
-          if (it >= hi) break;
-          const uint32_t u = it;
+          const uint32_t u = part + it * np;
+          if (u >= d) break;
           unsigned long long R = word_lane ? sh.rows[u * RS + lane] : 0ull;
           uint32_t l = 1, c = kdeep - 2;
           unsigned long long item_total = 0;
@@ -956,9 +954,7 @@ __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restric
           }
         }
       } else if constexpr (K == 3) {
-        const uint32_t lo = (uint32_t)((uint64_t)part * d / np);
-        const uint32_t hi = (uint32_t)((uint64_t)(part + 1) * d / np);
-        for (uint32_t u = lo + tid; u < hi; u += NT) {
+        for (uint32_t u = part + tid * np; u < d; u += NT * np) {
           uint32_t c = 0;
 #pragma unroll
           for (int j = 0; j < W; ++j) c += __popcll(sh.rows[u * RS + j]);
@@ -999,9 +995,8 @@ __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restric
           __syncthreads();
           nitems = sh.pre[d];
         }
-        const uint32_t ilo = (uint32_t)((uint64_t)part * nitems / np);
-        const uint32_t ihi = (uint32_t)((uint64_t)(part + 1) * nitems / np);
-        if (tid == 0) sh.next_item = ilo;
+        // next_item counts this part's items: the j-th is part + j np.
+        if (tid == 0) sh.next_item = 0;
         __syncthreads();
 
         // Dense roots (ranked kernels, expected |P| of an item ~ rho^2 d / 3
@@ -1030,8 +1025,8 @@ __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restric
             for (;;) {
               uint32_t it = 0;
               if (lane == 0) it = atomicAdd(&sh.next_item, 1u);
-              it = (uint32_t)__builtin_amdgcn_readfirstlane((int)it);
-              if (it >= ihi) break;
+              it = part + (uint32_t)__builtin_amdgcn_readfirstlane((int)it) * np;
+              if (it >= nitems) break;
               uint32_t lo = ucur, hi = d;  // pre[u] <= it < pre[u+1]
               while (hi - lo > 1) {
                 const uint32_t mid = (lo + hi) >> 1;
@@ -1090,7 +1085,7 @@ __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restric
           // item (measured faster there).  A lane's items are increasing,
           // which the search for u below relies on.
           constexpr bool kOnDemand = (K <= 5);
-          uint32_t e = ilo + tid;
+          uint32_t e = tid;  // static items: the lane's next index within the part
           auto fetch = [&]() {
             live = false;
             for (;;) {
@@ -1106,7 +1101,8 @@ __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restric
                 it = e;
                 e += NT;
               }
-              if (it >= ihi) return;
+              it = part + it * np;
+              if (it >= nitems) return;
               if constexpr (kEdgeItems) {
                 uint32_t lo = ucur, hi = d;  // pre[u] <= it < pre[u+1]
                 while (hi - lo > 1) {
@@ -1149,42 +1145,87 @@ __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restric
             }
           };
 
-          fetch();
-          while (live) {
-            if (bits) {
-              const uint32_t w = wi * 64 + (uint32_t)__builtin_ctzll(bits);
-              bits &= bits - 1;
-              const unsigned long long *r = sh.rows + w * RS;
-              unsigned long long Q[W];
+          // One step: the next w of the lane's set P.
+          auto step = [&]() {
+            const uint32_t w = wi * 64 + (uint32_t)__builtin_ctzll(bits);
+            bits &= bits - 1;
+            const unsigned long long *r = sh.rows + w * RS;
+            unsigned long long Q[W];
 #pragma unroll
-              for (int j = 0; j < W; ++j) Q[j] = P[j] & r[j];
-              if constexpr (ALL) {
-                const uint64_t c = cliques_all<W, kStepDepth>(Q, sh.rows, sh.cred);
-                if (c) atomicAdd(&sh.cred[w], (unsigned long long)c);
-                acc += c;
-              } else {
-                acc += cliques_in<W, kStepDepth>(Q, sh.rows);
-              }
-            } else if (wi + 1 < (uint32_t)W) {
-              ++wi;
-              unsigned long long nb = 0;
-#pragma unroll
-              for (int j = 0; j < W; ++j) nb = (j == (int)wi) ? P[j] : nb;
-              bits = nb;
+            for (int j = 0; j < W; ++j) Q[j] = P[j] & r[j];
+            if constexpr (ALL) {
+              const uint64_t c = cliques_all<W, kStepDepth>(Q, sh.rows, sh.cred);
+              if (c) atomicAdd(&sh.cred[w], (unsigned long long)c);
+              acc += c;
             } else {
-              if (acc) {
-                if constexpr (ALL) {
-                  atomicAdd(&sh.cred[u], acc);
-                  if constexpr (kEdgeItems) atomicAdd(&sh.cred[vsec], acc);
-                  root_acc += acc;
-                } else {
-                  const bool wins = kRanked ? (u < sh.nwin) : (sh.key[u] <= ks);
-                  if (!to_root && wins) atomicAdd(&sh.cred[u], acc);
-                  else root_acc += acc;
-                }
-                acc = 0;
+              acc += cliques_in<W, kStepDepth>(Q, sh.rows);
+            }
+          };
+          auto next_word = [&]() {
+            ++wi;
+            unsigned long long nb = 0;
+#pragma unroll
+            for (int j = 0; j < W; ++j) nb = (j == (int)wi) ? P[j] : nb;
+            bits = nb;
+          };
+          // The item's cliques: to its first member or to the root.
+          auto finish_item = [&]() {
+            if (acc) {
+              if constexpr (ALL) {
+                atomicAdd(&sh.cred[u], acc);
+                if constexpr (kEdgeItems) atomicAdd(&sh.cred[vsec], acc);
+                root_acc += acc;
+              } else {
+                const bool wins = kRanked ? (u < sh.nwin) : (sh.key[u] <= ks);
+                if (!to_root && wins) atomicAdd(&sh.cred[u], acc);
+                else root_acc += acc;
               }
-              fetch();
+              acc = 0;
+            }
+          };
+
+          fetch();
+          if constexpr (kOnDemand) {
+            // Steps are a few instructions and a fetch (item search, bit
+            // select, the set P) is several steps long, so lanes that run
+            // out of work wait until kRefillAt of them can fetch together:
+            // a round of steps then keeps most lanes busy instead of paying
+            // a near-empty fetch branch in every round.  A lane that drew
+            // past the last item is done (the queue only grows).
+            constexpr uint32_t kRefillAt = 16;
+            bool done = !live;
+            for (;;) {
+              for (;;) {
+                if (live && !bits) {
+                  while (!bits && wi + 1 < (uint32_t)W) next_word();
+                  if (!bits) {
+                    finish_item();
+                    live = false;
+                  }
+                }
+                const bool waiting = !live && !done;
+                if (__ballot(live) == 0ull ||
+                    (uint32_t)__popcll(__ballot(waiting)) >= kRefillAt)
+                  break;
+                if (live) step();
+              }
+              const bool waiting = !live && !done;
+              if (__ballot(waiting) == 0ull) break;
+              if (waiting) {
+                fetch();
+                done = !live;
+              }
+            }
+          } else {
+            while (live) {
+              if (bits) {
+                step();
+              } else if (wi + 1 < (uint32_t)W) {
+                next_word();
+              } else {
+                finish_item();
+                fetch();
+              }
             }
           }
         }
