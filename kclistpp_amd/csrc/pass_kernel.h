@@ -27,8 +27,8 @@
 //   k = 4: item u,       P = A'[u],          steps w in P: popcount(P & A'[w])
 //   k >= 5: item (u, v),  P = A'[u] & A'[v], steps w in P: cliques of size
 //           k-4 inside P & A'[w] (nested loops, compile-time depth)
-// A lane that finishes an item takes its next one at once, so lanes stay
-// busy until the root's items run out.  Credits gather in LDS (u64 per
+// Lanes that finish their items draw new ones together, once enough of them
+// wait, so that rounds of steps keep most lanes busy.  Credits gather in LDS (u64 per
 // member) and are flushed with one global atomicAdd per non-zero member,
 // plus one wave-reduced atomicAdd for the root.
 //
@@ -1185,14 +1185,18 @@ __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restric
           };
 
           fetch();
-          if constexpr (kOnDemand) {
+          {
             // Steps are a few instructions and a fetch (item search, bit
             // select, the set P) is several steps long, so lanes that run
             // out of work wait until kRefillAt of them can fetch together:
             // a round of steps then keeps most lanes busy instead of paying
-            // a near-empty fetch branch in every round.  A lane that drew
-            // past the last item is done (the queue only grows).
-            constexpr uint32_t kRefillAt = 16;
+            // a near-empty fetch branch in every round (measured at k = 5 on
+            // hub roots: 8, 16, 32, 48 waiting lanes give passes of 2.73,
+            // 2.57, 2.50, 2.55 ms).  A lane that drew past the last item is
+            // done (on demand the queue only grows; static items are the
+            // lane's own sequence).  Also measured at k = 6 (cl4m pass
+            // 3209 -> 3187 ms).
+            constexpr uint32_t kRefillAt = 32;
             bool done = !live;
             for (;;) {
               for (;;) {
@@ -1214,17 +1218,6 @@ __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restric
               if (waiting) {
                 fetch();
                 done = !live;
-              }
-            }
-          } else {
-            while (live) {
-              if (bits) {
-                step();
-              } else if (wi + 1 < (uint32_t)W) {
-                next_word();
-              } else {
-                finish_item();
-                fetch();
               }
             }
           }
