@@ -869,11 +869,16 @@ def secondary_leg(args, kcl, gg, name: str, steps: int = 10, warmup: int = 2):
     return out
 
 
-def roofline_of(workload: str, alg_bytes: int, kern_avg_ms: float) -> dict:
+def roofline_of(workload: str, alg_bytes: int, kern_avg_ms: float, dev: dict | None = None) -> dict:
     """The dominant kernel's roofline: algorithmic HBM bytes per pass over
     the enumeration kernels' time per pass (HIP events on the pass's
     stream), plus what the committed counter profile of this workload
-    measured (HBM traffic, VALU and LDS busy fractions, the bound)."""
+    measured (HBM traffic, VALU and LDS busy fractions, the bound).
+
+    valu_issue: the pass's VALU instructions (SQ_INSTS_VALU of one profiled
+    pass, wavefront instructions) at 4 cycles each on 4 SIMDs per CU at the
+    device's peak clock give the shortest pass the VALU can issue; frac is
+    that time over the measured one."""
     achieved = alg_bytes / (kern_avg_ms * 1e-3) / 1e9 if kern_avg_ms > 0 else 0.0
     traffic = profiled(workload, "traffic_bytes_per_pass")
     # The roof: HBM, the only one the integer pass has (it issues no MFMA).
@@ -892,6 +897,13 @@ def roofline_of(workload: str, alg_bytes: int, kern_avg_ms: float) -> dict:
         v = profiled(workload, key)
         if v is not None:
             out[key] = v
+    insts = profiled(workload, "valu_insts_per_pass")
+    if insts and dev and dev.get("cu_count") and dev.get("clock_khz") and kern_avg_ms > 0:
+        floor_ms = insts * 4.0 / (4.0 * dev["cu_count"]) / (dev["clock_khz"] * 1e3) * 1e3
+        out["valu_issue"] = {"insts_per_pass": insts, "cycles_per_inst": 4,
+                             "simds": 4 * dev["cu_count"], "clock_mhz": dev["clock_khz"] / 1e3,
+                             "floor_ms": round(floor_ms, 3),
+                             "frac": round(floor_ms / kern_avg_ms, 4)}
     return out
 
 
@@ -924,7 +936,7 @@ def run_sim(args, kcl, gg):
     ms_per_step = 1000.0 * elapsed / max(args.steps, 1)
     value = visits / elapsed if elapsed > 0 else 0.0
     kern_avg_ms = kern_ms / max(args.steps, 1)
-    roofline = roofline_of(args.workload, alg_bytes, kern_avg_ms)
+    roofline = roofline_of(args.workload, alg_bytes, kern_avg_ms, kcl.device_info())
     plain = None
     if args.plain_steps > 0 and args.schedule != "degree":
         t, plain = plain_leg(kcl, g, k, sim, t, args.plain_steps)
