@@ -27,16 +27,20 @@
 //   k = 4: item u,       P = A'[u],          steps w in P: popcount(P & A'[w])
 //   k >= 5: item (u, v),  P = A'[u] & A'[v], steps w in P: cliques of size
 //           k-4 inside P & A'[w] (nested loops, compile-time depth)
-// Lanes that finish their items draw new ones together, once enough of them
-// wait, so that rounds of steps keep most lanes busy.  Credits gather in LDS (u64 per
+// At k <= 5, lanes that finish their items draw new ones together once
+// enough of them wait (steps are short, so rounds of steps then keep most
+// lanes busy); at k >= 6 a lane draws at once.  Credits gather in LDS (u64 per
 // member) and are flushed with one global atomicAdd per non-zero member,
 // plus one wave-reduced atomicAdd for the root.
 //
-// Heavy roots are split into parts: part p of np takes items p, p + np,
-// p + 2 np, ... and rebuilds A' (the rebuild is cheap next to the counting
-// it splits).  Strided parts see the same mix of items: in ranked kernels
-// an item's cost grows steeply with its members' ranks, so contiguous
-// ranges would leave one part with most of the root's work.
+// Heavy roots are split into parts, each rebuilding A' (the rebuild is cheap
+// next to the counting it splits).  Items are dealt out in blocks of
+// kPartBlock(NW) consecutive items: part p of np takes blocks p, p + np,
+// p + 2 np, ...  Dealt-out blocks give every part the same mix of items (in
+// ranked kernels an item's cost grows steeply with its members' ranks, so
+// contiguous ranges leave one part with most of a root's work), and a block
+// of neighbours keeps every wavefront of the part busy on the heavy items
+// at its start.
 //
 // Dense roots (k >= 6).  The k >= 6 kernels of rows of 4+ words number
 // the members by their (key, id) rank before building A', so A'[x] holds only ranks above x
@@ -413,6 +417,14 @@ __device__ __forceinline__ void wave_scan(uint32_t *a, uint32_t d, uint32_t lane
 
 // ALL: credit every member of every clique (k-clique degrees) instead of
 // the winner or the root.
+// Items per dealt-out block of a split root: 64 per wavefront.  Learned
+// passes (ms) on 1M/16M k = 6, 4M/60M k = 6, R-MAT scales 15 and 16 k = 7
+// at one state, by block size: 1 (items dealt one by one) 16.8, 3621, 2271,
+// 6830; 4 NW 15.7, 3614, 1828, 6057; 16 NW 15.3, 3608, 1640, 5659;
+// 64 NW 15.2, 3610, 1437, 5562; 256 NW 16.2, 3605, 1667, 6428; one
+// contiguous range per part 15.7, 3644, 1708, 6506.
+constexpr uint32_t kPartBlock(uint32_t nw) { return 64u * nw; }
+
 template <int W, int K, int NW, bool ALL>
 __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restrict__ cd,
                                                        const uint32_t *__restrict__ adj,
@@ -499,6 +511,9 @@ __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restric
     const bool edge_task = task_is_edge(tk.part_np);
     const uint32_t part = edge_task ? 0u : (tk.part_np & 0xffffu);
     const uint32_t np = edge_task ? 1u : (tk.part_np >> 16);
+    // The j-th item of this part (increasing in j; np = 1: item j).
+    constexpr uint32_t kBlk = kPartBlock(NW);
+    auto part_item = [&](uint32_t j) { return ((j / kBlk) * np + part) * kBlk + j % kBlk; };
     uint32_t d;
     uint64_t ks;
     uint32_t eu = s;  // edge tasks: the second root node u
@@ -591,7 +606,9 @@ __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restric
     unsigned long long root_acc = 0;
     if constexpr (K == 2) {
       // Cliques {s, u}: items are the members themselves.
-      for (uint32_t u = part + tid * np; u < d; u += NT * np) {
+      for (uint32_t j = tid;; j += NT) {
+        const uint32_t u = part_item(j);
+        if (u >= d) break;
         if constexpr (ALL) {
           sh.cred[u] += 1;
           root_acc += 1;
@@ -734,7 +751,7 @@ __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restric
         // chosen member's set is final and its edges are counted (or
         // listed) by the lanes.  Items of kdeep <= 4 (emit passes only)
         // stop at the item's own set.
-        if (tid == 0) sh.next_item = 0;  // the j-th item of this part is part + j np
+        if (tid == 0) sh.next_item = 0;  // counts this part's items (part_item)
         __syncthreads();
         const uint32_t wid = tid >> 6;
         const uint32_t levels = kdeep > 3 ? kdeep - 3 : 1u;  // path / count slots, 1-based
@@ -807,7 +824,7 @@ __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restric
           uint32_t it = 0;
           if (lane == 0) it = atomicAdd(&sh.next_item, 1u);
           it = (uint32_t)__builtin_amdgcn_readfirstlane((int)it);
-          const uint32_t u = part + it * np;
+          const uint32_t u = part_item(it);
           if (u >= d) break;
           unsigned long long R = word_lane ? sh.rows[u * RS + lane] : 0ull;
           uint32_t l = 1, c = kdeep - 2;
@@ -954,7 +971,9 @@ __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restric
           }
         }
       } else if constexpr (K == 3) {
-        for (uint32_t u = part + tid * np; u < d; u += NT * np) {
+        for (uint32_t j = tid;; j += NT) {
+          const uint32_t u = part_item(j);
+          if (u >= d) break;
           uint32_t c = 0;
 #pragma unroll
           for (int j = 0; j < W; ++j) c += __popcll(sh.rows[u * RS + j]);
@@ -995,7 +1014,7 @@ __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restric
           __syncthreads();
           nitems = sh.pre[d];
         }
-        // next_item counts this part's items: the j-th is part + j np.
+        // next_item counts this part's items (part_item).
         if (tid == 0) sh.next_item = 0;
         __syncthreads();
 
@@ -1025,7 +1044,7 @@ __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restric
             for (;;) {
               uint32_t it = 0;
               if (lane == 0) it = atomicAdd(&sh.next_item, 1u);
-              it = part + (uint32_t)__builtin_amdgcn_readfirstlane((int)it) * np;
+              it = part_item((uint32_t)__builtin_amdgcn_readfirstlane((int)it));
               if (it >= nitems) break;
               uint32_t lo = ucur, hi = d;  // pre[u] <= it < pre[u+1]
               while (hi - lo > 1) {
@@ -1101,7 +1120,7 @@ __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restric
                 it = e;
                 e += NT;
               }
-              it = part + it * np;
+              it = part_item(it);
               if (it >= nitems) return;
               if constexpr (kEdgeItems) {
                 uint32_t lo = ucur, hi = d;  // pre[u] <= it < pre[u+1]
@@ -1185,7 +1204,7 @@ __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restric
           };
 
           fetch();
-          {
+          if constexpr (kOnDemand) {
             // Steps are a few instructions and a fetch (item search, bit
             // select, the set P) is several steps long, so lanes that run
             // out of work wait until kRefillAt of them can fetch together:
@@ -1193,9 +1212,7 @@ __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restric
             // a near-empty fetch branch in every round (measured at k = 5 on
             // hub roots: 8, 16, 32, 48 waiting lanes give passes of 2.73,
             // 2.57, 2.50, 2.55 ms).  A lane that drew past the last item is
-            // done (on demand the queue only grows; static items are the
-            // lane's own sequence).  Also measured at k = 6 (cl4m pass
-            // 3209 -> 3187 ms).
+            // done (the queue only grows).
             constexpr uint32_t kRefillAt = 32;
             bool done = !live;
             for (;;) {
@@ -1218,6 +1235,23 @@ __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restric
               if (waiting) {
                 fetch();
                 done = !live;
+              }
+            }
+          }
+          else {
+            // k >= 6: a lane refills at once.  Its steps are nested loops of
+            // their own, of uneven length, so lanes waiting for a batch
+            // would idle longer than a lone fetch costs (batched at 32:
+            // 1M/16M k = 6 pass 16.8 -> 17.6 ms, R-MAT scale 15 k = 7
+            // 2310 -> 2340 ms, 4M/60M k = 6 3621 -> 3600 ms).
+            while (live) {
+              if (bits) {
+                step();
+              } else if (wi + 1 < (uint32_t)W) {
+                next_word();
+              } else {
+                finish_item();
+                fetch();
               }
             }
           }
