@@ -89,8 +89,8 @@ constexpr uint32_t class_waves(uint32_t c, uint32_t k, bool all = false) {
 constexpr uint32_t kMaxWords = 16;
 constexpr uint32_t kMaxRootDegree = 64 * kMaxWords;
 
-// One root task: root node s, split part p of np (items [p*I/np, (p+1)*I/np)),
-// or an edge task (root edge (s, u), u < 2^31, flagged in the top bit).
+// One root task: root node s, split part p of np (the root's items dealt out
+// in blocks, part p taking blocks p, p + np, ...), or an edge task (root edge (s, u), u < 2^31, flagged in the top bit).
 struct Task {
   uint32_t s;
   uint32_t part_np;  // part | (np << 16), or kEdgeTaskFlag | u
