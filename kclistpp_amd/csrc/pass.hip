@@ -63,10 +63,21 @@ uint32_t max_pass_k() { return kMaxDeepK; }
 
 namespace {
 
-uint32_t blocks_per_cu(uint32_t wclass, const DeviceProps &props, uint32_t waves) {
+// Resident workgroups per CU of a class's kernel: as many as LDS and the 32
+// wave slots allow, except that the 1- and 2-word classes of counting passes
+// at k <= 5 keep to 16 waves per CU.  Their roots wait on chains of global
+// loads while holding wave slots, and at k <= 5 the wider classes that run
+// beside them are VALU bound and need those slots: the 1M/16M k = 5 pass
+// takes 2.47 ms with 32 light waves per CU, 1.96 ms with 16 or 8.  At k = 6
+// the cap cost 2.7 % (1M/16M) and was neutral on 4M/60M.
+uint32_t blocks_per_cu(uint32_t wclass, const DeviceProps &props, uint32_t waves, uint32_t k,
+                       bool all) {
   uint32_t per_cu = (uint32_t)(props.lds_per_cu / (lds_bytes(wclass) + 256));
   if (per_cu < 1) per_cu = 1;
   if (per_cu > 32 / waves) per_cu = 32 / waves;
+  constexpr uint32_t kLightWaves = 16;
+  if (!all && k <= 5 && wclass <= 1 && per_cu * waves > kLightWaves)
+    per_cu = std::max(1u, kLightWaves / waves);
   return per_cu;
 }
 
@@ -96,7 +107,7 @@ kcl_status class_launch_shape(uint32_t wclass, uint32_t k, bool all, uint32_t ma
   if (lc >= kNumClasses) return fail(KCL_E_UNSUPPORTED, "k = %u: no word class holds the walk", k);
   wclass = lc;
   const uint32_t w = waves_for(wclass, k, all);
-  uint32_t pc = blocks_per_cu(wclass, props, w);
+  uint32_t pc = blocks_per_cu(wclass, props, w, k, all);
   if (max_per_cu && max_per_cu < pc) pc = max_per_cu;
   uint64_t b = (uint64_t)props.cu_count * pc;
   if (b > count) b = count;
@@ -110,7 +121,7 @@ kcl_status class_launch_shape(uint32_t wclass, uint32_t k, bool all, uint32_t ma
 uint32_t class_slots(uint32_t wclass, uint32_t k) {
   DeviceProps props;
   if (wclass >= kNumClasses || device_props(&props) != KCL_OK) return 1;
-  return props.cu_count * blocks_per_cu(wclass, props, waves_for(wclass, k, false));
+  return props.cu_count * blocks_per_cu(wclass, props, waves_for(wclass, k, false), k, false);
 }
 
 kcl_status launch_pass(const kcl_graph *g, const uint64_t *d_key, uint32_t k, PassMode mode,
@@ -168,7 +179,7 @@ kcl_status launch_pass_raw(const kcl_graph *g, const uint64_t *d_key, uint32_t k
       // Emit passes: prof is the emit block, shared by every class.
       unsigned long long *prof_arg =
           mode == kEmit ? prof : (prof ? prof + 2 * (size_t)begin[c] : nullptr);
-      uint32_t per_cu = blocks_per_cu(lc, props, waves);
+      uint32_t per_cu = blocks_per_cu(lc, props, waves, kk, all);
       if (plan.max_per_cu && plan.max_per_cu < per_cu) per_cu = plan.max_per_cu;
       uint64_t blocks = (uint64_t)props.cu_count * per_cu;
       if (blocks > count) blocks = count;
