@@ -327,13 +327,19 @@ def test_counts_do_not_depend_on_dense_threshold(kcl):
         b.close()
 
 
-@pytest.mark.parametrize("split", [1, 7, 100])
-def test_counts_do_not_depend_on_splitting(kcl, split):
-    pairs = _community_graph(5, size=60, p=0.9)
-    _say("split", split)
+@pytest.mark.parametrize("k,split,size", [(5, 1, 60), (5, 7, 60), (5, 100, 60), (3, 1, 60),
+                                          (4, 1, 60), (6, 1, 60), (11, 1, 30)])
+def test_counts_do_not_depend_on_splitting(kcl, k, split, size):
+    """Split roots deal their items (members at k <= 4 and k > 10, edges
+    otherwise) out to their parts in blocks; with split_work = 1 small roots
+    get more parts than blocks of the full size, and every item must still
+    be counted once."""
+    pairs = _community_graph(5, size=size, p=0.9)
+    _say("split", split, "k", k)
     with kcl.Graph(pairs) as g:
-        base = kcl.Sim(g, 5)
-        other = kcl.Sim(g, 5, split_work=split)
+        base = kcl.Sim(g, k)
+        other = kcl.Sim(g, k, split_work=split)
+        _say("tasks", base.plan_info()["tasks"], "split into", other.plan_info()["tasks"])
         for t in (1, 2, 3):
             base.step(t)
             other.step(t)
