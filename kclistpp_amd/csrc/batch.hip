@@ -225,6 +225,7 @@ kcl_status apply_plan_config(kcl_batch::KPlan *kp, uint32_t min_class, uint32_t 
   promote_plan(&p, min_class);
   KCL_TRY(upload_plan(p, &kp->dev));
   kp->dev.max_per_cu = per_cu;
+  kp->dev.light_waves = 32;  // residency is what the autotuner tunes here
   return KCL_OK;
 }
 

@@ -87,6 +87,9 @@ constexpr uint32_t class_waves(uint32_t c, uint32_t k, bool all = false) {
                                               : (k <= 5 ? 8u : 4u);
 }
 constexpr uint32_t kMaxWords = 16;
+// Waves per CU of the 1- and 2-word classes in counting passes at k <= 5
+// (pass.hip: blocks_per_cu); 32 lifts the cap.
+constexpr uint32_t kLightWaves = 16;
 constexpr uint32_t kMaxRootDegree = 64 * kMaxWords;
 
 // One root task: root node s, split part p of np (the root's items dealt out
@@ -162,6 +165,7 @@ struct DevicePlan {
   uint32_t eclass_count[kNumClasses] = {};
   uint32_t light_from[kNumClasses] = {};
   uint32_t max_per_cu = 0;  // cap on resident workgroups per CU (0: by LDS)
+  uint32_t light_waves = kLightWaves;  // waves per CU of the 1-2 word classes at k <= 5
 };
 
 struct DeviceProps {
@@ -194,6 +198,7 @@ struct PassLaunch {
   const uint32_t *eclass_count = nullptr;
   const uint32_t *light_from = nullptr;
   uint32_t max_per_cu = 0;
+  uint32_t light_waves = kLightWaves;
   // Vertex tasks take their members from this device list (ascending ids,
   // at most the class's 64 W) instead of N+(s); nullptr: N+(s).
   const uint32_t *list = nullptr;

@@ -69,15 +69,16 @@ namespace {
 // loads while holding wave slots, and at k <= 5 the wider classes that run
 // beside them are VALU bound and need those slots: the 1M/16M k = 5 pass
 // takes 2.47 ms with 32 light waves per CU, 1.96 ms with 16 or 8.  At k = 6
-// the cap cost 2.7 % (1M/16M) and was neutral on 4M/60M.
+// the cap cost 2.7 % (1M/16M) and was neutral on 4M/60M.  light_waves is the
+// plan's cap (kLightWaves by default; the batch, whose passes are nearly all
+// light roots, lifts it and tunes residency itself).
 uint32_t blocks_per_cu(uint32_t wclass, const DeviceProps &props, uint32_t waves, uint32_t k,
-                       bool all) {
+                       bool all, uint32_t light_waves = kLightWaves) {
   uint32_t per_cu = (uint32_t)(props.lds_per_cu / (lds_bytes(wclass) + 256));
   if (per_cu < 1) per_cu = 1;
   if (per_cu > 32 / waves) per_cu = 32 / waves;
-  constexpr uint32_t kLightWaves = 16;
-  if (!all && k <= 5 && wclass <= 1 && per_cu * waves > kLightWaves)
-    per_cu = std::max(1u, kLightWaves / waves);
+  if (!all && k <= 5 && wclass <= 1 && per_cu * waves > light_waves)
+    per_cu = std::max(1u, light_waves / waves);
   return per_cu;
 }
 
@@ -136,6 +137,7 @@ kcl_status launch_pass(const kcl_graph *g, const uint64_t *d_key, uint32_t k, Pa
   pl.eclass_count = plan.eclass_count;
   pl.light_from = plan.light_from;
   pl.max_per_cu = plan.max_per_cu;
+  pl.light_waves = plan.light_waves;
   return launch_pass_raw(g, d_key, k, mode, pl, d_cnt, stream, prof);
 }
 
@@ -179,7 +181,7 @@ kcl_status launch_pass_raw(const kcl_graph *g, const uint64_t *d_key, uint32_t k
       // Emit passes: prof is the emit block, shared by every class.
       unsigned long long *prof_arg =
           mode == kEmit ? prof : (prof ? prof + 2 * (size_t)begin[c] : nullptr);
-      uint32_t per_cu = blocks_per_cu(lc, props, waves, kk, all);
+      uint32_t per_cu = blocks_per_cu(lc, props, waves, kk, all, plan.light_waves);
       if (plan.max_per_cu && plan.max_per_cu < per_cu) per_cu = plan.max_per_cu;
       uint64_t blocks = (uint64_t)props.cu_count * per_cu;
       if (blocks > count) blocks = count;

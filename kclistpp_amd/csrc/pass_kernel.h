@@ -35,8 +35,8 @@
 //
 // Heavy roots are split into parts, each rebuilding A' (the rebuild is cheap
 // next to the counting it splits).  Items are dealt out in blocks of
-// kPartBlock(NW) consecutive items (fewer when the root has too few items
-// for min(np, 8) parts to get one): part p of np takes blocks p, p + np,
+// kPartBlock(NW) consecutive items (member items: fewer, so that min(np, 8)
+// parts get some): part p of np takes blocks p, p + np,
 // p + 2 np, ...  Dealt-out blocks give every part the same mix of items (in
 // ranked kernels an item's cost grows steeply with its members' ranks, so
 // contiguous ranges leave one part with most of a root's work), and a block
@@ -512,18 +512,26 @@ __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restric
     const bool edge_task = task_is_edge(tk.part_np);
     const uint32_t part = edge_task ? 0u : (tk.part_np & 0xffffu);
     const uint32_t np = edge_task ? 1u : (tk.part_np >> 16);
-    // The j-th item of this part, items dealt out in blocks of blk
-    // (increasing in j; np = 1: item j).  part_block(n): the block for a root
-    // of n items, kPartBlock(NW), or less when fewer than min(np, 8) parts
-    // would get items (every part computes it from the same n).  Parts past
-    // the last block rebuild the rows and stop: on R-MAT scale 15 at k = 7,
-    // shrinking blocks until every part had items was 5 % slower.
-    auto part_item = [&](uint32_t j, uint32_t blk) {
-      return np == 1 ? j : ((j / blk) * np + part) * blk + j % blk;
+    // The j-th item of this part (increasing in j; np = 1: item j).  Edge
+    // items (k >= 5) are dealt out in blocks of kPartBlock(NW); parts past
+    // the last block rebuild the rows and stop.  Member items (k <= 4 and the
+    // walk kernels) come in blocks of 2^sh, the largest power of two up to
+    // kPartBlock(NW) that still gives min(np, 8) parts items (part_shift;
+    // every part computes it from the same n): a root of at most 64 W members
+    // would otherwise sit in one block.  Shrinking the edge-item blocks the
+    // same way made the 1M/16M k = 5 pass 13 % slower, and shrinking them
+    // until every part had items cost 5 % on R-MAT scale 15 at k = 7.
+    constexpr uint32_t kBlk = kPartBlock(NW);
+    auto part_edge_item = [&](uint32_t j) {
+      return np == 1 ? j : ((j / kBlk) * np + part) * kBlk + j % kBlk;
     };
-    auto part_block = [&](uint32_t n) {
+    auto part_item = [&](uint32_t j, uint32_t sh2) {
+      return np == 1 ? j : ((((j >> sh2) * np + part) << sh2) | (j & ((1u << sh2) - 1u)));
+    };
+    auto part_shift = [&](uint32_t n) {
       const uint32_t useful = min(np, 8u);
-      return max(1u, min(kPartBlock(NW), (n + useful - 1) / useful));
+      const uint32_t b = max(1u, min(kPartBlock(NW), (n + useful - 1) / useful));
+      return 31u - (uint32_t)__builtin_clz(b);
     };
     uint32_t d;
     uint64_t ks;
@@ -617,9 +625,9 @@ __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restric
     unsigned long long root_acc = 0;
     if constexpr (K == 2) {
       // Cliques {s, u}: items are the members themselves.
-      const uint32_t blk = part_block(d);
+      const uint32_t bsh = part_shift(d);
       for (uint32_t j = tid;; j += NT) {
-        const uint32_t u = part_item(j, blk);
+        const uint32_t u = part_item(j, bsh);
         if (u >= d) break;
         if constexpr (ALL) {
           sh.cred[u] += 1;
@@ -764,7 +772,7 @@ __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restric
         // listed) by the lanes.  Items of kdeep <= 4 (emit passes only)
         // stop at the item's own set.
         if (tid == 0) sh.next_item = 0;  // counts this part's items (part_item)
-        const uint32_t blk = part_block(d);
+        const uint32_t bsh = part_shift(d);
         __syncthreads();
         const uint32_t wid = tid >> 6;
         const uint32_t levels = kdeep > 3 ? kdeep - 3 : 1u;  // path / count slots, 1-based
@@ -837,7 +845,7 @@ __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restric
           uint32_t it = 0;
           if (lane == 0) it = atomicAdd(&sh.next_item, 1u);
           it = (uint32_t)__builtin_amdgcn_readfirstlane((int)it);
-          const uint32_t u = part_item(it, blk);
+          const uint32_t u = part_item(it, bsh);
           if (u >= d) break;
           unsigned long long R = word_lane ? sh.rows[u * RS + lane] : 0ull;
           uint32_t l = 1, c = kdeep - 2;
@@ -984,9 +992,9 @@ __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restric
           }
         }
       } else if constexpr (K == 3) {
-        const uint32_t blk = part_block(d);
+        const uint32_t bsh = part_shift(d);
         for (uint32_t j = tid;; j += NT) {
-          const uint32_t u = part_item(j, blk);
+          const uint32_t u = part_item(j, bsh);
           if (u >= d) break;
           uint32_t c = 0;
 #pragma unroll
@@ -1029,7 +1037,7 @@ __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restric
           nitems = sh.pre[d];
         }
         // next_item counts this part's items (part_item).
-        const uint32_t iblk = part_block(nitems);
+        const uint32_t ibsh = part_shift(nitems);
         if (tid == 0) sh.next_item = 0;
         __syncthreads();
 
@@ -1059,7 +1067,8 @@ __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restric
             for (;;) {
               uint32_t it = 0;
               if (lane == 0) it = atomicAdd(&sh.next_item, 1u);
-              it = part_item((uint32_t)__builtin_amdgcn_readfirstlane((int)it), iblk);
+              it = (uint32_t)__builtin_amdgcn_readfirstlane((int)it);
+              it = kEdgeItems ? part_edge_item(it) : part_item(it, ibsh);
               if (it >= nitems) break;
               uint32_t lo = ucur, hi = d;  // pre[u] <= it < pre[u+1]
               while (hi - lo > 1) {
@@ -1135,7 +1144,7 @@ __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restric
                 it = e;
                 e += NT;
               }
-              it = part_item(it, iblk);
+              it = kEdgeItems ? part_edge_item(it) : part_item(it, ibsh);
               if (it >= nitems) return;
               if constexpr (kEdgeItems) {
                 uint32_t lo = ucur, hi = d;  // pre[u] <= it < pre[u+1]
