@@ -43,6 +43,10 @@ WORKLOADS = {
                "R-MAT (0.57, 0.19, 0.19, 0.05) scale 15, m=2e8/256 before dedup, k=7"),
     "rmat16": ("rmat23", 1, 128, 7,
                "R-MAT (0.57, 0.19, 0.19, 0.05) scale 16, m=2e8/128 before dedup, k=7"),
+    "rmat21": ("rmat23", 1, 4, 7,
+               "R-MAT (0.57, 0.19, 0.19, 0.05) scale 21, m=2e8/4 before dedup, k=7"),
+    "rmat23": ("rmat23", 1, 1, 7,
+               "R-MAT (0.57, 0.19, 0.19, 0.05) scale 23, m=2e8 before dedup, k=7"),
 }
 
 
@@ -681,6 +685,62 @@ def run_cmpsync(args, kcl, gg):
     print(json.dumps(out), flush=True)
 
 
+def run_input(args, kcl, gg):
+    """The input path at scale: text edge file -> reader -> degeneracy order
+    -> relabel -> CSR (kcl_read_edge_list + kcl_graph_create), wall clock.
+    cpu_baseline: the reference's own ReadEdgeList + SortByCore + Relabel +
+    MakeGraph (Graph.c:34-156, built from its sources, single-threaded as
+    written) on the same file; its ranks and DAG edges are compared with
+    this build's bit for bit."""
+    wname, seed, sd, k, desc = WORKLOADS[args.workload]
+    log(f"bench input: generating {args.workload} ({desc})")
+    t = time.time()
+    pairs = workload_pairs(args, gg, wname, seed, sd)
+    log(f"bench input: {pairs.shape[0]} edges in {time.time() - t:.1f}s")
+    six = np.array([(a, b) for a in range(6) for b in range(a + 1, 6)], np.uint32)
+    with kcl.Graph(six):  # loads the library's kernels outside the timed region
+        pass
+    with tempfile.TemporaryDirectory() as td:
+        path = os.path.join(td, "edges.txt")
+        gg.write_text(pairs, path, progress=lambda i: log(f"bench input: {i} lines written"))
+        m_in = int(pairs.shape[0])
+        del pairs
+        log(f"bench input: text file {os.path.getsize(path) / 1e6:.0f} MB")
+        t0 = time.perf_counter()
+        pr, n_in = kcl.read_edge_list(path)
+        t1 = time.perf_counter()
+        g = kcl.Graph(pr, n_in)
+        t2 = time.perf_counter()
+        del pr
+        info = {"n_in": n_in, "n": g.n, "m": g.m, "core": g.core}
+        log(f"bench input: read {t1 - t0:.2f}s, graph {t2 - t1:.2f}s, {info}")
+        cpu, parity = None, None
+        if not args.no_cpu_baseline:
+            from oracle import RefHarness
+            ref = RefHarness()
+            l0 = time.perf_counter()
+            h = ref.load(path)
+            load_s = time.perf_counter() - l0
+            log(f"bench input: reference load {load_s:.2f}s")
+            parity = {"rank_equal": bool(np.array_equal(g.rank(), ref.rank(h))),
+                      "edges_equal": bool(np.array_equal(g.edges(), ref.edges(h)))}
+            ref.free(h)
+            cpu = {"value": load_s, "unit": "s", "cores": 1, "kind": "reference",
+                   "cpu_model": cpu_model(),
+                   "sample": "the reference's ReadEdgeList + SortByCore + Relabel + MakeGraph "
+                             "(built from its sources, single-threaded as written) on the same "
+                             f"text file: {load_s:.2f}s"}
+        g.close()
+    out = {"metric": "input path time (read + degeneracy order + relabel + CSR)",
+           "value": t2 - t0, "unit": "s", "n_gpus": 1, "higher_is_better": False,
+           "vs_baseline": None, "dtype": "u32",
+           "data": "synthetic (seeded generator, kclistpp_amd/graphgen.py)",
+           "config": {"workload": f"{args.workload}: {desc}", "edges_in_file": m_in, **info},
+           "phases_s": {"read": t1 - t0, "graph": t2 - t1}, "parity": parity,
+           "cpu_baseline": cpu}
+    print(json.dumps(out), flush=True)
+
+
 def run_exact(args, kcl, gg):
     """Time-to-exact: text edge file -> graph -> certified maximal densest
     subgraph (kcl_exact_run), wall clock."""
@@ -747,12 +807,14 @@ def main():
     ap.add_argument("--exact-budget", type=int, default=240,
                     help="exact mode: seconds before giving up on a certificate")
     ap.add_argument("--mode", default="sim",
-                    choices=["sim", "exact", "trim", "makespan", "sampling", "cmpsync"],
+                    choices=["sim", "exact", "trim", "makespan", "sampling", "cmpsync",
+                             "input"],
                     help="sim: visits/s of one iteration (default); exact: time-to-exact; "
                          "trim: k-clique core trimming time; makespan: each schedule's "
                          "pass against a makespan lower bound; sampling: Sampling++ time "
                          "(--steps rounds); cmpsync: kCDensestCmpSync end to end "
-                         "(T = --steps)")
+                         "(T = --steps); input: reader + graph build against the "
+                         "reference's loader")
     ap.add_argument("--gpu-peel-min", type=int, default=0,
                     help="trim mode: pops through at least this many cliques go to the GPU "
                          "(0: the library's default)")
@@ -795,6 +857,9 @@ def main():
         return
     if args.mode == "cmpsync":
         run_cmpsync(args, kcl, gg)
+        return
+    if args.mode == "input":
+        run_input(args, kcl, gg)
         return
     if args.workload == "batch512":
         run_batch(args, kcl, gg)

@@ -193,8 +193,9 @@ def finalize(pairs: np.ndarray, rng: np.random.Generator) -> np.ndarray:
     return np.ascontiguousarray(out, dtype=np.uint32)
 
 
-def write_text(pairs: np.ndarray, path: str) -> None:
-    """The reference's input format: one ``"%u %u\\n"`` line per edge."""
+def write_text(pairs: np.ndarray, path: str, progress=None) -> None:
+    """The reference's input format: one ``"%u %u\\n"`` line per edge.
+    progress(lines_written) is called after every 2^24 lines."""
     with open(path, "w") as f:
         if pairs.shape[0] == 0:
             return
@@ -203,6 +204,8 @@ def write_text(pairs: np.ndarray, path: str) -> None:
             blk = pairs[i:i + step]
             f.write("\n".join(f"{a} {b}" for a, b in blk.tolist()))
             f.write("\n")
+            if progress and ((i + step) >> 24) != (i >> 24):
+                progress(min(i + step, pairs.shape[0]))
 
 
 def write_binary(pairs: np.ndarray, path: str) -> None:
