@@ -8,8 +8,10 @@
 // smaller); each neighbour of the popped node, in edge-list order, is
 // decreased by one and sifted up (move while the parent is strictly
 // larger); rank = n - (pop index + 1).
+#include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 #include "kcl_internal.h"
@@ -61,24 +63,79 @@ struct Heap {
 
 }  // namespace
 
-void degeneracy_rank(uint32_t n, uint64_t m, const uint32_t *pairs, uint32_t *rank) {
-  if (n == 0) return;
-  // Undirected adjacency in edge order (Graph.c:62-74).
-  std::vector<uint64_t> off((size_t)n + 1, 0);
-  for (uint64_t i = 0; i < m; ++i) {
-    off[pairs[2 * i] + 1]++;
-    off[pairs[2 * i + 1] + 1]++;
-  }
-  for (uint32_t v = 0; v < n; ++v) off[v + 1] += off[v];
-  std::vector<uint32_t> nbr(2 * m + 1);
-  {
+namespace {
+
+// Undirected adjacency in edge order (Graph.c:62-74): for each edge (s, t) in
+// turn, t is appended to s's list, then s to t's.  Threads take contiguous
+// chunks of the edge list: each counts its chunk's entries per node, a
+// per-node prefix over the threads places chunk c's entries after those of
+// chunks before it, and each thread then writes its own entries in order, so
+// every list is the sequential one.
+void undirected_adjacency(uint32_t n, uint64_t m, const uint32_t *pairs,
+                          std::vector<uint64_t> *off_out, std::vector<uint32_t> *nbr_out) {
+  std::vector<uint64_t> &off = *off_out;
+  std::vector<uint32_t> &nbr = *nbr_out;
+  off.assign((size_t)n + 1, 0);
+  nbr.assign(2 * m + 1, 0);
+  unsigned T = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  if (m < (uint64_t(1) << 22)) T = 1;
+  if (T == 1) {
+    for (uint64_t i = 0; i < m; ++i) {
+      off[pairs[2 * i] + 1]++;
+      off[pairs[2 * i + 1] + 1]++;
+    }
+    for (uint32_t v = 0; v < n; ++v) off[v + 1] += off[v];
     std::vector<uint64_t> fill(off.begin(), off.end() - 1);
     for (uint64_t i = 0; i < m; ++i) {
       uint32_t s = pairs[2 * i], t = pairs[2 * i + 1];
       nbr[fill[s]++] = t;
       nbr[fill[t]++] = s;
     }
+    return;
   }
+  // cnt[c][v]: entries of node v in chunk c (then: entries of v in chunks < c).
+  std::vector<std::vector<uint32_t>> cnt(T, std::vector<uint32_t>(n, 0));
+  auto run = [&](auto &&body) {
+    std::vector<std::thread> th;
+    for (unsigned c = 0; c < T; ++c) th.emplace_back(body, c);
+    for (auto &x : th) x.join();
+  };
+  run([&](unsigned c) {
+    uint32_t *k = cnt[c].data();
+    for (uint64_t i = m * c / T, e = m * (c + 1) / T; i < e; ++i) {
+      k[pairs[2 * i]]++;
+      k[pairs[2 * i + 1]]++;
+    }
+  });
+  run([&](unsigned c) {  // per node: exclusive prefix over the chunks, total degree
+    for (uint64_t v = (uint64_t)n * c / T, e = (uint64_t)n * (c + 1) / T; v < e; ++v) {
+      uint64_t run_sum = 0;
+      for (unsigned q = 0; q < T; ++q) {
+        const uint32_t k = cnt[q][v];
+        cnt[q][v] = (uint32_t)run_sum;
+        run_sum += k;
+      }
+      off[v + 1] = run_sum;
+    }
+  });
+  for (uint32_t v = 0; v < n; ++v) off[v + 1] += off[v];
+  run([&](unsigned c) {
+    uint32_t *k = cnt[c].data();
+    for (uint64_t i = m * c / T, e = m * (c + 1) / T; i < e; ++i) {
+      uint32_t s = pairs[2 * i], t = pairs[2 * i + 1];
+      nbr[off[s] + k[s]++] = t;
+      nbr[off[t] + k[t]++] = s;
+    }
+  });
+}
+
+}  // namespace
+
+void degeneracy_rank(uint32_t n, uint64_t m, const uint32_t *pairs, uint32_t *rank) {
+  if (n == 0) return;
+  std::vector<uint64_t> off;
+  std::vector<uint32_t> nbr;
+  undirected_adjacency(n, m, pairs, &off, &nbr);
   Heap h;
   h.a.resize(n);
   h.where.assign(n, UINT32_MAX);
