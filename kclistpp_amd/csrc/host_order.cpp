@@ -78,7 +78,9 @@ void undirected_adjacency(uint32_t n, uint64_t m, const uint32_t *pairs,
   off.assign((size_t)n + 1, 0);
   nbr.assign(2 * m + 1, 0);
   unsigned T = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-  if (m < (uint64_t(1) << 22)) T = 1;
+  // Small graphs, and sparse ids (the per-chunk counts take T n words, so
+  // they must stay small next to the 2 m adjacency words), stay sequential.
+  if (m < (uint64_t(1) << 22) || (uint64_t)T * n > 2 * m) T = 1;
   if (T == 1) {
     for (uint64_t i = 0; i < m; ++i) {
       off[pairs[2 * i] + 1]++;
