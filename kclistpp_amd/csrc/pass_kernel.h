@@ -416,8 +416,6 @@ __device__ __forceinline__ void wave_scan(uint32_t *a, uint32_t d, uint32_t lane
   }
 }
 
-// ALL: credit every member of every clique (k-clique degrees) instead of
-// the winner or the root.
 // Items per dealt-out block of a split root: 64 per wavefront.  Learned
 // passes (ms) on 1M/16M k = 6, 4M/60M k = 6, R-MAT scales 15 and 16 k = 7
 // at one state, by block size: 1 (items dealt one by one) 16.8, 3621, 2271,
@@ -426,6 +424,8 @@ __device__ __forceinline__ void wave_scan(uint32_t *a, uint32_t d, uint32_t lane
 // contiguous range per part 15.7, 3644, 1708, 6506.
 constexpr uint32_t kPartBlock(uint32_t nw) { return 64u * nw; }
 
+// ALL: credit every member of every clique (k-clique degrees) instead of
+// the winner or the root.
 template <int W, int K, int NW, bool ALL>
 __global__ __launch_bounds__(64 * NW) void pass_kernel(const uint32_t *__restrict__ cd,
                                                        const uint32_t *__restrict__ adj,
