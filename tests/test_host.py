@@ -182,3 +182,41 @@ def test_certificate_max_flow_matches_edmonds_karp(tmp_path):
                        env=dict(os.environ, CLIQUE_FLOW_COMMUNITY="60"))
     assert r.returncode == 0, r.stdout + r.stderr
     assert "mismatches: 0" in r.stdout and "saturated 1" in r.stdout
+
+
+@pytest.mark.parametrize("case", ["threaded", "sparse_ids", "small"])
+def test_host_degeneracy_order_matches_oracle(oracle, tmp_path, case):
+    """The host replay of the reference's heap (host_order.cpp) against the
+    oracle's restatement, on a graph large enough for the threaded
+    adjacency build (m >= 2^22, ids dense), one whose sparse ids keep it
+    sequential, and a small one."""
+    import shutil
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("g++ not available")
+    from kclistpp_amd import graphgen as gg
+    if case == "threaded":
+        pairs = gg.workload("cl1m", seed=4, scale_down=3)
+        assert pairs.shape[0] >= (1 << 22)
+    elif case == "sparse_ids":
+        rng = np.random.default_rng(9)
+        ids = np.sort(rng.choice(200_000_000, 100_000, replace=False)).astype(np.uint32)
+        pairs = gg.workload("er20k", seed=3, scale_down=1)
+        pairs = np.ascontiguousarray(ids[pairs % ids.size])
+        pairs = pairs[pairs[:, 0] != pairs[:, 1]]
+    else:
+        pairs = gg.workload("er20k", seed=5, scale_down=1)
+    exe = str(tmp_path / "dgc")
+    subprocess.run([gxx, "-O2", "-std=c++17", "-pthread",
+                    "-I", os.path.join(ROOT, "kclistpp_amd", "csrc"), "-I", os.path.join(ROOT, "include"),
+                    "-I", "/opt/rocm/include", "-D__HIP_PLATFORM_AMD__",
+                    os.path.join(ROOT, "tests", "native", "degeneracy_check.cpp"),
+                    os.path.join(ROOT, "kclistpp_amd", "csrc", "host_order.cpp"), "-o", exe],
+                   check=True)
+    src, dst = str(tmp_path / "pairs.u32"), str(tmp_path / "rank.u32")
+    np.ascontiguousarray(pairs, "<u4").tofile(src)
+    subprocess.run([exe, src, dst], check=True, timeout=300)
+    got = np.fromfile(dst, "<u4")
+    n = int(pairs.max()) + 1
+    assert got.size == n
+    assert np.array_equal(got, oracle.rank(pairs, n))
